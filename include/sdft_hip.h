@@ -1,0 +1,167 @@
+/* C ABI of the MI355X (gfx950) backend for Sparse-DFT's plane-wave H.psi /
+ * S.psi application and block-Davidson step.
+ *
+ * Conventions: complex values are interleaved FP64 (re, im) pairs, matrices
+ * are column-major with explicit leading dimensions counted in complex
+ * elements, exactly the host layouts of the reference's
+ * src/compute_backend.hpp:13-27 (Complex64 = std::complex<double>).
+ * Every function returns a status: 0 success, 1 runtime error
+ * (std::runtime_error), 2 invalid argument (std::invalid_argument), 3 overflow
+ * (std::overflow_error); the message is in sdft_hip_last_error() of the
+ * calling thread. A context belongs to one host thread, as the reference keeps
+ * one thread_local Accelerator per thread (src/compute_backend.cpp:62-73).
+ * precision: 0 = FP64 arithmetic, 1 = FP32 arithmetic (search precision).
+ */
+#ifndef SDFT_HIP_H
+#define SDFT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sdft_hip_context sdft_hip_context;
+typedef struct sdft_hip_operator sdft_hip_operator;
+
+#define SDFT_HIP_OK 0
+#define SDFT_HIP_RUNTIME_ERROR 1
+#define SDFT_HIP_INVALID_ARGUMENT 2
+#define SDFT_HIP_OVERFLOW 3
+
+/* Thread-local message of the last failed call ("HIP: ..."). */
+const char* sdft_hip_last_error(void);
+
+/* Capability JSON; replaces compiled_compute_capabilities()
+ * (src/compute_backend.cpp:110-124) with "hip_compiled": true. No GPU needed. */
+int sdft_hip_capabilities(char* buffer, size_t size);
+
+/* Context (HIP stream + rocBLAS/rocSOLVER handle + grow-only buffers); the
+ * state a `hip` Accelerator owns (src/compute_cuda.cu:82-85 for CUDA). */
+int sdft_hip_context_create(int device, sdft_hip_context** out);
+int sdft_hip_context_destroy(sdft_hip_context* ctx);
+/* Accelerator::device() (src/compute_backend.hpp:66) */
+int sdft_hip_device_name(sdft_hip_context* ctx, char* buffer, size_t size);
+int sdft_hip_synchronize(sdft_hip_context* ctx);
+
+/* Accelerator::gemm / gemm_immutable_left (src/compute_backend.hpp:67-73):
+ * C = op(A) op(B) (+ C if add), ops 0 = Normal, 1 = Adjoint, host pointers.
+ * left_identity != 0 keeps op(A) resident on the device for that identity
+ * (src/compute_backend.hpp:32-41). Never writes outside m x n of ldc. */
+int sdft_hip_zgemm(sdft_hip_context* ctx, int aop, int bop, size_t m, size_t n, size_t k, const double* a, size_t lda,
+                   const double* b, size_t ldb, double* c, size_t ldc, int add, uint64_t left_identity,
+                   int precision);
+
+/* Accelerator::gemm_triple_right (src/compute_backend.hpp:74-80):
+ * C_i = A_i B for i = 0..2, normal operands, host pointers. */
+int sdft_hip_zgemm_triple_right(sdft_hip_context* ctx, size_t m, size_t n, size_t k, const double* const a[3],
+                                size_t lda, const double* b, size_t ldb, double* const c[3], size_t ldc,
+                                int precision);
+
+/* Accelerator::local_hamiltonian (src/compute_backend.hpp:81-83):
+ * out[:,c] = T in[:,c] + FFT_fwd(V(r) FFT_bwd(in[:,c])) / N for `columns`
+ * contiguous columns of npw values; indices into the row-major grid `shape`,
+ * potential = V(r) on the grid (N values). operator_identity != 0 keeps
+ * shape/indices/potential resident (src/compute_backend.hpp:44-45). */
+int sdft_hip_local_hamiltonian(sdft_hip_context* ctx, const int shape[3], const uint64_t* indices, size_t npw,
+                               const double* kinetic, const double* potential, const double* input, size_t columns,
+                               double* output, uint64_t operator_identity, int precision);
+
+/* Accelerator::paw_pair (src/compute_backend.hpp:88-89; CUDA chain
+ * src/compute_cuda.cu:95-128): h = B^H D B in, s = B^H Q B in with B
+ * (projectors x npw), D, Q (projectors x projectors, column-major). Only the
+ * nonlocal parts are returned. projector_identity guards B, identity guards
+ * D and Q. Returns 0 and sets *handled = 1 when computed. */
+int sdft_hip_paw_pair(sdft_hip_context* ctx, uint64_t identity, size_t npw, size_t projectors, size_t columns,
+                      const double* b, const double* d, const double* q, const double* input, double* h, double* s,
+                      uint64_t projector_identity, int precision, int* handled);
+
+/* G+k sphere of src/pw_core.cpp:140-171 for the real-space lattice rows
+ * (bohr, reciprocal per src/scf_core.cpp:169-175), cutoff in eV. Writes at
+ * most `capacity` entries; *count receives the sphere size. Host only. */
+int sdft_hip_enumerate_g_vectors(const double lattice[9], double cutoff_ev, const double kfrac[3],
+                                 double ev_per_hartree, long capacity, int* hkl, double* kinetic_ev,
+                                 double* cartesian, long* count);
+
+/* Row-major grid index of src/plane_wave_operator.cpp:94-97. Host only. */
+int sdft_hip_grid_indices(const int shape[3], size_t n, const int* hkl, uint64_t* indices);
+
+/* Device-resident plane-wave operator: the PlaneWaveOperator constructor of
+ * src/plane_wave_operator.cpp:393-516 on the device (index map, V(r) from
+ * V(G) by a device FFT, diagonals, stacked projectors, Hermitised D/Q).
+ * shape[a] <= 0 selects the reference grid rule (fast_fft_length). */
+typedef struct {
+  size_t n;                 /* plane waves */
+  const int* hkl;           /* n x 3, reference G order */
+  const double* kinetic_ev; /* n */
+  double ev_per_hartree;
+  size_t nv;                /* Fourier potential entries */
+  const int* v_hkl;         /* nv x 3 */
+  const double* v_values;   /* nv complex */
+  size_t atoms;
+  const long* rows;         /* projectors per atom */
+  const double* amplitudes; /* per atom row-major rows x n complex, concatenated */
+  const double* d;          /* per atom row-major rows x rows complex, concatenated */
+  const double* q;
+  double regularization;
+  int shape[3];
+} sdft_hip_operator_desc;
+
+int sdft_hip_operator_create(sdft_hip_context* ctx, const sdft_hip_operator_desc* desc, sdft_hip_operator** out);
+int sdft_hip_operator_destroy(sdft_hip_operator* op);
+/* shape, grid size, projectors, sticks; arrays may be NULL (host outputs). */
+int sdft_hip_operator_info(sdft_hip_operator* op, int shape[3], long* count, long* projectors, long* sticks,
+                           uint64_t* indices, double* kinetic, double* local_real, double* h_diagonal,
+                           double* s_diagonal);
+/* HermitianOperator::apply_block_into / apply_pair_into
+ * (src/pw_core.hpp:119-130): which 0 = H, 1 = S, 2 = joint H and S.
+ * device_pointers = 1: x, hy, sy are device pointers (n x columns). */
+int sdft_hip_operator_apply(sdft_hip_operator* op, int which, const double* x, size_t columns, double* hy, double* sy,
+                            int precision, int device_pointers);
+/* Averages `repetitions` device-resident applications of `columns` columns
+ * with HIP events on the context stream (inputs resident in HBM).
+ * kernel_ms[0..2]: local-H pipeline, projector GEMMs + onsite, whole apply. */
+int sdft_hip_operator_benchmark(sdft_hip_operator* op, int which, size_t columns, int precision, int repetitions,
+                                double* kernel_ms);
+
+/* solve_generalized_hermitian_davidson (src/pw_core.cpp:532-1437), device
+ * resident. search_precision 1 runs H/S applies and subspace products in FP32
+ * until the hand-off to FP64; Rayleigh-Ritz, orthonormalisation and the
+ * certificate are FP64. */
+typedef struct {
+  long roots;
+  long max_iterations;
+  double tolerance;
+  long subspace_factor;
+  int search_precision;
+  double search_handoff;
+  int stagnation_window;
+  int verbose;
+  int timing;
+} sdft_hip_davidson_params;
+
+typedef struct {
+  double* eigenvalues;    /* roots, caller-allocated */
+  double* eigenvectors;   /* n x roots complex, caller-allocated or NULL */
+  double* residual_norms; /* roots */
+  long iterations;
+  int converged;
+  long h_columns_search;
+  long h_columns_exact;
+  long handoff_iteration;
+  double seconds_total, seconds_apply, seconds_gemm, seconds_ritz, seconds_ortho;
+} sdft_hip_davidson_result;
+
+int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* params, const double* initial,
+                      sdft_hip_davidson_result* result);
+
+/* Compensated per-atom onsite products W = [D_a P_a | Q_a P_a] on the device
+ * (src/paw_compensated_product.hpp:110-135 order), host pointers; for tests. */
+int sdft_hip_onsite_blocks(sdft_hip_context* ctx, size_t atoms, const long* rows, const double* d, const double* q,
+                           const double* p, size_t columns, double* w, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

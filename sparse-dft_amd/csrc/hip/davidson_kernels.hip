@@ -1,0 +1,328 @@
+// Column-wise reductions and elementwise steps of the block Davidson
+// (src/pw_core.cpp:532-1437) on device-resident blocks. Blocks are column-major
+// complex FP64 with leading dimension ld (complex elements).
+#include <hip/hip_runtime.h>
+
+#include "davidson_kernels.hpp"
+
+namespace sdft_hip {
+
+namespace {
+constexpr int TPB = 256;
+
+// error-free transformation a + b = s + e
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+#pragma clang fp contract(off)
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+// double-double accumulate (hi, lo) += (h2, l2)
+__device__ __forceinline__ void dd_add(double& hi, double& lo, double h2, double l2) {
+#pragma clang fp contract(off)
+  double s, e;
+  two_sum(hi, h2, s, e);
+  e += lo + l2;
+  two_sum(s, e, hi, lo);
+}
+
+__device__ __forceinline__ void block_reduce_dd(double& hi, double& lo, double* sh, double* sl) {
+  const int t = threadIdx.x;
+  sh[t] = hi;
+  sl[t] = lo;
+  __syncthreads();
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      double h = sh[t], l = sl[t];
+      dd_add(h, l, sh[t + s], sl[t + s]);
+      sh[t] = h;
+      sl[t] = l;
+    }
+    __syncthreads();
+  }
+  hi = sh[0];
+  lo = sl[0];
+}
+
+// one block per column: sum_i re(a_i) re(b_i) + im(a_i) im(b_i), compensated
+// (FMA-exact products accumulated in double-double; fixed reduction order)
+__global__ __launch_bounds__(TPB) void k_real_dots(const double* __restrict__ a, long lda,
+                                                   const double* __restrict__ b, long ldb, long n,
+                                                   double* __restrict__ out) {
+#pragma clang fp contract(off)
+  __shared__ double sh[TPB], sl[TPB];
+  const long col = blockIdx.x;
+  const double* ac = a + 2 * col * lda;
+  const double* bc = b + 2 * col * ldb;
+  double hi = 0, lo = 0;
+  for (long i = threadIdx.x; i < n; i += TPB) {
+    const double ar = ac[2 * i], ai = ac[2 * i + 1], br = bc[2 * i], bi = bc[2 * i + 1];
+    const double p1 = ar * br, p2 = ai * bi;
+    dd_add(hi, lo, p1, __builtin_fma(ar, br, -p1));
+    dd_add(hi, lo, p2, __builtin_fma(ai, bi, -p2));
+  }
+  block_reduce_dd(hi, lo, sh, sl);
+  if (threadIdx.x == 0) out[col] = hi + lo;
+}
+
+// one block per (i, j): <a_i, b_j> = sum conj(a) b, compensated
+__global__ __launch_bounds__(TPB) void k_complex_dots(const double* __restrict__ a, long lda,
+                                                      const double* __restrict__ b, long ldb, long n, int ca,
+                                                      double* __restrict__ out, long ldo) {
+#pragma clang fp contract(off)
+  __shared__ double sh[TPB], sl[TPB];
+  const long i = blockIdx.x, j = blockIdx.y;
+  const double* ac = a + 2 * i * lda;
+  const double* bc = b + 2 * j * ldb;
+  double rh = 0, rl = 0, ih = 0, il = 0;
+  for (long k = threadIdx.x; k < n; k += TPB) {
+    const double ar = ac[2 * k], ai = ac[2 * k + 1], br = bc[2 * k], bi = bc[2 * k + 1];
+    const double p1 = ar * br, p2 = ai * bi, p3 = ar * bi, p4 = ai * br;
+    dd_add(rh, rl, p1, __builtin_fma(ar, br, -p1));
+    dd_add(rh, rl, p2, __builtin_fma(ai, bi, -p2));
+    dd_add(ih, il, p3, __builtin_fma(ar, bi, -p3));
+    dd_add(ih, il, -p4, -__builtin_fma(ai, br, -p4));
+  }
+  block_reduce_dd(rh, rl, sh, sl);
+  __syncthreads();
+  block_reduce_dd(ih, il, sh, sl);
+  if (threadIdx.x == 0) {
+    double* o = out + 2 * (i + j * ldo);
+    o[0] = rh + rl;
+    o[1] = ih + il;
+  }
+  (void)ca;
+}
+
+// per column: sqrt(sum |x|^2)
+__global__ __launch_bounds__(TPB) void k_col_norms(const double* __restrict__ x, long ldx, long n,
+                                                   double* __restrict__ out) {
+  __shared__ double sh[TPB];
+  const long col = blockIdx.x;
+  const double* xc = x + 2 * col * ldx;
+  double s = 0;
+  for (long i = threadIdx.x; i < n; i += TPB) s += xc[2 * i] * xc[2 * i] + xc[2 * i + 1] * xc[2 * i + 1];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = TPB / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[col] = sqrt(sh[0]);
+}
+
+// per column: sum_i |x_i|^2 T_i
+__global__ __launch_bounds__(TPB) void k_kinetic_weights(const double* __restrict__ x, long ldx, long n,
+                                                         const double* __restrict__ kinetic,
+                                                         double* __restrict__ out) {
+  __shared__ double sh[TPB];
+  const long col = blockIdx.x;
+  const double* xc = x + 2 * col * ldx;
+  double s = 0;
+  for (long i = threadIdx.x; i < n; i += TPB) s += (xc[2 * i] * xc[2 * i] + xc[2 * i + 1] * xc[2 * i + 1]) * kinetic[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = TPB / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[col] = sh[0];
+}
+
+// src/pw_core.cpp:33-47
+__device__ __forceinline__ double kinetic_preconditioner(double kinetic, double wk) {
+  const double scale = 1.5 * fmax(wk, 2. / 27.211386245988);
+  const double x = kinetic / scale;
+  if (x > 1.) {
+    const double inv = 1. / x;
+    const double p = inv * (8. + inv * (12. + inv * (18. + 27. * inv)));
+    return (2. / scale) * p / (p + 16.);
+  }
+  const double p = 27. + x * (18. + x * (12. + 8. * x));
+  return (2. / scale) * p / (p + 16. * x * x * x * x);
+}
+
+// out(:, j) = -K(T, wk_j) * r(:, src[j])
+__global__ void k_precondition(const double* __restrict__ r, long ldr, const int* __restrict__ src,
+                               const double* __restrict__ wk, const double* __restrict__ kinetic, long n,
+                               double* __restrict__ out, long ldo) {
+  const long col = blockIdx.y;
+  const int sc = src[col];
+  const double w = wk[sc];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double k = -kinetic_preconditioner(kinetic[i], w);
+    out[2 * (i + col * ldo)] = r[2 * (i + sc * ldr)] * k;
+    out[2 * (i + col * ldo) + 1] = r[2 * (i + sc * ldr) + 1] * k;
+  }
+}
+
+// residual = hx - sx * diag(values)   (Ritz step)
+__global__ void k_ritz_residual(const double* __restrict__ hx, const double* __restrict__ sx, long ld, long n,
+                                const double* __restrict__ values, double* __restrict__ r) {
+  const long col = blockIdx.y;
+  const double v = values[col];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long k = 2 * (i + col * ld);
+    r[k] = hx[k] - sx[k] * v;
+    r[k + 1] = hx[k + 1] - sx[k + 1] * v;
+  }
+}
+
+// residual = fma(-value, sx, hx) per element (src/pw_core.cpp:587-593)
+__global__ void k_fma_residual(const double* __restrict__ hx, const double* __restrict__ sx, long ld, long n,
+                               const double* __restrict__ values, double* __restrict__ r) {
+  const long col = blockIdx.y;
+  const double v = values[col];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long k = 2 * (i + col * ld);
+    r[k] = __builtin_fma(-v, sx[k], hx[k]);
+    r[k + 1] = __builtin_fma(-v, sx[k + 1], hx[k + 1]);
+  }
+}
+
+// x(:, j) *= s_j
+__global__ void k_scale_columns(double* __restrict__ x, long ld, long n, const double* __restrict__ s) {
+  const long col = blockIdx.y;
+  const double f = s[col];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    x[2 * (i + col * ld)] *= f;
+    x[2 * (i + col * ld) + 1] *= f;
+  }
+}
+
+// x(:, j) /= s_j  (division, as the reference divides by the norm)
+__global__ void k_divide_columns(double* __restrict__ x, long ld, long n, const double* __restrict__ s) {
+  const long col = blockIdx.y;
+  const double f = s[col];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    x[2 * (i + col * ld)] /= f;
+    x[2 * (i + col * ld) + 1] /= f;
+  }
+}
+
+// trial(:, j) = fma(alpha_j, d(:, j), x(:, j)) where alpha_j != 0, else x
+__global__ void k_polish_update(const double* __restrict__ x, const double* __restrict__ d, long ld, long n,
+                                const double* __restrict__ alpha, double* __restrict__ t) {
+  const long col = blockIdx.y;
+  const double a = alpha[col];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long k = 2 * (i + col * ld);
+    if (a != 0.0) {
+      t[k] = __builtin_fma(a, d[k], x[k]);
+      t[k + 1] = __builtin_fma(a, d[k + 1], x[k + 1]);
+    } else {
+      t[k] = x[k];
+      t[k + 1] = x[k + 1];
+    }
+  }
+}
+
+// M(0:s, 0:s) <- (M + M^H) / 2, written into out (ld_out)
+__global__ void k_hermitise(const double* __restrict__ m, long ldm, long s, double* __restrict__ out, long ldo) {
+  const long total = s * s;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const long i = e % s, j = e / s;
+    const double ar = m[2 * (i + j * ldm)], ai = m[2 * (i + j * ldm) + 1];
+    const double br = m[2 * (j + i * ldm)], bi = -m[2 * (j + i * ldm) + 1];
+    out[2 * (i + j * ldo)] = 0.5 * (ar + br);
+    out[2 * (i + j * ldo) + 1] = 0.5 * (ai + bi);
+  }
+}
+
+// U(:, j) *= 1 / sqrt(lambda_j)  (block CGS2 eigen-normalisation)
+__global__ void k_inv_sqrt_columns(double* __restrict__ u, long ld, long rows, const double* __restrict__ lambda) {
+  const long col = blockIdx.y;
+  const double f = 1. / sqrt(lambda[col]);
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < rows; i += (long)gridDim.x * blockDim.x) {
+    u[2 * (i + col * ld)] *= f;
+    u[2 * (i + col * ld) + 1] *= f;
+  }
+}
+
+__global__ void k_finite_check(const double* __restrict__ x, long ld, long n, long cols, int* __restrict__ bad) {
+  const long total = n * cols;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const long i = e % n, j = e / n;
+    const double a = x[2 * (i + j * ld)], b = x[2 * (i + j * ld) + 1];
+    if (!isfinite(a) || !isfinite(b)) *bad = 1;
+  }
+}
+
+dim3 grid_cols(long n, long cols) {
+  long bx = (n + 255) / 256;
+  if (bx > 64) bx = 64;
+  return dim3(unsigned(bx), unsigned(cols));
+}
+}  // namespace
+
+#define SDFT_CHECKED(call) \
+  do {                     \
+    call;                  \
+    return hipGetLastError(); \
+  } while (0)
+
+hipError_t launch_real_dots(const double* a, long lda, const double* b, long ldb, long n, long cols, double* out,
+                            hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_real_dots, dim3(unsigned(cols)), dim3(TPB), 0, s, a, lda, b, ldb, n, out));
+}
+hipError_t launch_complex_dots(const double* a, long lda, long ca, const double* b, long ldb, long cb, long n,
+                               double* out, long ldo, hipStream_t s) {
+  if (ca <= 0 || cb <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_complex_dots, dim3(unsigned(ca), unsigned(cb)), dim3(TPB), 0, s, a, lda, b, ldb,
+                                  n, 0, out, ldo));
+}
+hipError_t launch_col_norms(const double* x, long ldx, long n, long cols, double* out, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_col_norms, dim3(unsigned(cols)), dim3(TPB), 0, s, x, ldx, n, out));
+}
+hipError_t launch_kinetic_weights(const double* x, long ldx, long n, long cols, const double* kinetic, double* out,
+                                  hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_kinetic_weights, dim3(unsigned(cols)), dim3(TPB), 0, s, x, ldx, n, kinetic, out));
+}
+hipError_t launch_precondition(const double* r, long ldr, const int* src, long cols, const double* wk,
+                               const double* kinetic, long n, double* out, long ldo, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_precondition, grid_cols(n, cols), dim3(256), 0, s, r, ldr, src, wk, kinetic, n,
+                                  out, ldo));
+}
+hipError_t launch_ritz_residual(const double* hx, const double* sx, long ld, long n, long cols, const double* values,
+                                double* r, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_ritz_residual, grid_cols(n, cols), dim3(256), 0, s, hx, sx, ld, n, values, r));
+}
+hipError_t launch_fma_residual(const double* hx, const double* sx, long ld, long n, long cols, const double* values,
+                               double* r, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_fma_residual, grid_cols(n, cols), dim3(256), 0, s, hx, sx, ld, n, values, r));
+}
+hipError_t launch_scale_columns(double* x, long ld, long n, long cols, const double* scales, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_scale_columns, grid_cols(n, cols), dim3(256), 0, s, x, ld, n, scales));
+}
+hipError_t launch_divide_columns(double* x, long ld, long n, long cols, const double* divisors, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_divide_columns, grid_cols(n, cols), dim3(256), 0, s, x, ld, n, divisors));
+}
+hipError_t launch_polish_update(const double* x, const double* d, long ld, long n, long cols, const double* alpha,
+                                double* trial, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_polish_update, grid_cols(n, cols), dim3(256), 0, s, x, d, ld, n, alpha, trial));
+}
+hipError_t launch_hermitise(const double* m, long ldm, long size, double* out, long ldo, hipStream_t s) {
+  if (size <= 0) return hipSuccess;
+  const long total = size * size;
+  const int blocks = int(total / 256 + 1 > 2048 ? 2048 : total / 256 + 1);
+  SDFT_CHECKED(hipLaunchKernelGGL(k_hermitise, dim3(blocks), dim3(256), 0, s, m, ldm, size, out, ldo));
+}
+hipError_t launch_inv_sqrt_columns(double* u, long ld, long rows, long cols, const double* lambda, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_inv_sqrt_columns, grid_cols(rows, cols), dim3(256), 0, s, u, ld, rows, lambda));
+}
+hipError_t launch_finite_check(const double* x, long ld, long n, long cols, int* bad, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_finite_check, dim3(512), dim3(256), 0, s, x, ld, n, cols, bad));
+}
+
+}  // namespace sdft_hip

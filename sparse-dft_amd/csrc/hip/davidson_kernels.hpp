@@ -1,0 +1,30 @@
+// Launchers of the Davidson column reductions and elementwise steps.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+namespace sdft_hip {
+
+// out[j] = sum_i Re(a_ij) Re(b_ij) + Im(a_ij) Im(b_ij), compensated
+hipError_t launch_real_dots(const double* a, long lda, const double* b, long ldb, long n, long cols, double* out,
+                            hipStream_t s);
+// out(i, j) = sum_k conj(a_ki) b_kj, compensated, for i < ca, j < cb
+hipError_t launch_complex_dots(const double* a, long lda, long ca, const double* b, long ldb, long cb, long n,
+                               double* out, long ldo, hipStream_t s);
+hipError_t launch_col_norms(const double* x, long ldx, long n, long cols, double* out, hipStream_t s);
+hipError_t launch_kinetic_weights(const double* x, long ldx, long n, long cols, const double* kinetic, double* out,
+                                  hipStream_t s);
+hipError_t launch_precondition(const double* r, long ldr, const int* src, long cols, const double* wk,
+                               const double* kinetic, long n, double* out, long ldo, hipStream_t s);
+hipError_t launch_ritz_residual(const double* hx, const double* sx, long ld, long n, long cols, const double* values,
+                                double* r, hipStream_t s);
+hipError_t launch_fma_residual(const double* hx, const double* sx, long ld, long n, long cols, const double* values,
+                               double* r, hipStream_t s);
+hipError_t launch_scale_columns(double* x, long ld, long n, long cols, const double* scales, hipStream_t s);
+hipError_t launch_divide_columns(double* x, long ld, long n, long cols, const double* divisors, hipStream_t s);
+hipError_t launch_polish_update(const double* x, const double* d, long ld, long n, long cols, const double* alpha,
+                                double* trial, hipStream_t s);
+hipError_t launch_hermitise(const double* m, long ldm, long size, double* out, long ldo, hipStream_t s);
+hipError_t launch_inv_sqrt_columns(double* u, long ld, long rows, long cols, const double* lambda, hipStream_t s);
+hipError_t launch_finite_check(const double* x, long ld, long n, long cols, int* bad, hipStream_t s);
+
+}  // namespace sdft_hip

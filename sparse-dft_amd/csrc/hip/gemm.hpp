@@ -1,0 +1,35 @@
+// Complex GEMM and PAW onsite-block launchers (MFMA, column-major, interleaved
+// complex storage in FP64 unless stated otherwise).
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+
+namespace sdft_hip {
+
+enum class Precision { FP64 = 0, FP32 = 1 };
+
+int zgemm_splits(int M, int N, int K);
+std::size_t zgemm_workspace_bytes(int M, int N, int K);
+
+// C = alpha op(A) op(B) + beta * D; aop/bop: 0 = Normal, 1 = Adjoint. D may
+// alias C. `prec` selects the arithmetic (FP64 or FP32 MFMA); storage is FP64.
+hipError_t zgemm(Precision prec, int aop, int bop, int M, int N, int K, double alpha, const double* A, long lda,
+                 const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta, double* work,
+                 std::size_t work_bytes, hipStream_t s);
+// Same with A stored in FP32 (resident search copy of the projectors).
+hipError_t zgemm_f32_storage(int aop, int bop, int M, int N, int K, double alpha, const float* A, long lda,
+                             const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta,
+                             double* work, std::size_t work_bytes, hipStream_t s);
+
+// W(:, 0:m) = D_a P_a and W(:, m:2m) = Q_a P_a per atom block (which_mask bit
+// 0: D, bit 1: Q). FP64 uses the compensated scalar order of the reference.
+hipError_t launch_atom_blocks(Precision prec, int np, int columns, int atoms, const int* atom_offset,
+                              const int* atom_rows, const long* block_offset, const double* dblk, const double* qblk,
+                              const double* P, long ldp, double* W, long ldw, int which_mask, hipStream_t s);
+
+// C = alpha X + beta D
+hipError_t launch_zaxpby(int M, int N, double alpha, const double* X, long ldx, double beta, const double* D, long ldd,
+                         double* C, long ldc, hipStream_t s);
+
+}  // namespace sdft_hip

@@ -1,0 +1,310 @@
+// Local Hamiltonian H_loc psi = T psi + FFT_fwd(V(r) . FFT_bwd(psi)) / N on
+// gfx950, restating src/plane_wave_operator.cpp:229-258 (and the CUDA chain of
+// src/compute_cuda.cu:141-177) as three launches that touch only the sphere:
+//
+//   A  k_sticks_backward : gather psi into the (x,y) sticks of the G-sphere,
+//                          z-FFT (natural -> slot order), store stick buffer
+//                          T1[column][z slot][stick]  (coalesced over sticks)
+//   B  k_planes          : per (z slot, column) xy-plane in LDS: scatter the
+//                          sticks, y- and x-FFT to slots, x V(r) in slot
+//                          order, x- and y-FFT back to natural order, keep
+//                          only the sticks -> T2
+//   C  k_sticks_forward  : z-FFT from slots to natural order, gather onto the
+//                          sphere, add the kinetic term and divide by N
+//
+// The full grid never exists in HBM; per column the traffic is the two stick
+// buffers (nsticks * N2 values each) plus psi in/out.
+#include <hip/hip_runtime.h>
+
+#include "fft_device.hpp"
+#include "fft_lengths.hpp"
+#include "local_h.hpp"
+
+namespace sdft_hip {
+
+namespace {
+
+template <typename T, int S, bool TO_SLOTS>
+__device__ __forceinline__ void line_fft_dispatch(int length, cx<T>* base, const LineGeom& g, const cx<T>* tw, int tid,
+                                                  int nt) {
+  switch (length) {
+#define SDFT_HIP_CASE(L, A, B)                                 \
+  case L:                                                      \
+    if (TO_SLOTS)                                              \
+      lds_fft_to_slots<A, B, S, T>(base, g, tw, tid, nt);      \
+    else                                                       \
+      lds_fft_from_slots<A, B, S, T>(base, g, tw, tid, nt);    \
+    break;
+    SDFT_HIP_FOR_EACH_LENGTH(SDFT_HIP_CASE)
+#undef SDFT_HIP_CASE
+    default:
+      break;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> load_cx(const double* p) {
+  const double2 v = *reinterpret_cast<const double2*>(p);
+  return {T(v.x), T(v.y)};
+}
+
+}  // namespace
+
+// A: grid (ceil(nsticks / spb), columns)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sticks_backward(const double* __restrict__ x, long ldx,
+                                                         const int* __restrict__ stick_g, int nsticks, int n2, int spb,
+                                                         const cx<T>* __restrict__ tw_z, cx<T>* __restrict__ t1) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lines = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int first = blockIdx.x * spb;
+  const int nl = min(spb, nsticks - first);
+  const long col = blockIdx.y;
+  const double* xc = x + 2 * col * ldx;
+  for (int e = tid; e < nl * n2; e += nt) {
+    const int s = e / n2, z = e - s * n2;
+    const int g = stick_g[(long)(first + s) * n2 + z];
+    lines[e] = g >= 0 ? load_cx<T>(xc + 2L * g) : cx<T>{T(0), T(0)};
+  }
+  __syncthreads();
+  LineGeom geom{nl, n2, 1, false};
+  line_fft_dispatch<T, +1, true>(n2, lines, geom, tw_z, tid, nt);
+  cx<T>* out = t1 + col * (long)n2 * nsticks;
+  for (int e = tid; e < nl * n2; e += nt) {
+    const int zs = e / nl, s = e - zs * nl;
+    out[(long)zs * nsticks + first + s] = lines[s * n2 + zs];
+  }
+}
+
+// B: grid (n2 slots, columns). mode 0: local H (bwd, x V, fwd, keep sticks);
+// mode 1: backward only, whole plane stored in slot order (potential setup).
+template <typename T>
+__global__ __launch_bounds__(256) void k_planes(const cx<T>* __restrict__ t1, cx<T>* __restrict__ t2,
+                                                const int* __restrict__ stick_xy, int nsticks, int n0, int n1, int n2,
+                                                const cx<T>* __restrict__ tw_x, const cx<T>* __restrict__ tw_y,
+                                                const T* __restrict__ v_slots, int mode) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* plane = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int zs = blockIdx.x;
+  const long col = blockIdx.y;
+  const int np = n0 * n1;
+  for (int e = tid; e < np; e += nt) plane[e] = {T(0), T(0)};
+  __syncthreads();
+  const cx<T>* in = t1 + (col * n2 + zs) * (long)nsticks;
+  for (int s = tid; s < nsticks; s += nt) plane[stick_xy[s]] = in[s];
+  __syncthreads();
+  // y lines: one per x, contiguous elements
+  const LineGeom gy{n0, n1, 1, false};
+  // x lines: one per y, elements strided by n1
+  const LineGeom gx{n1, 1, n1, true};
+  line_fft_dispatch<T, +1, true>(n1, plane, gy, tw_y, tid, nt);
+  line_fft_dispatch<T, +1, true>(n0, plane, gx, tw_x, tid, nt);
+  if (mode == 1) {
+    cx<T>* out = t2 + (col * n2 + zs) * (long)np;
+    for (int e = tid; e < np; e += nt) out[e] = plane[e];
+    return;
+  }
+  const T* v = v_slots + (long)zs * np;
+  for (int e = tid; e < np; e += nt) {
+    const T w = v[e];
+    plane[e].x *= w;
+    plane[e].y *= w;
+  }
+  __syncthreads();
+  line_fft_dispatch<T, -1, false>(n0, plane, gx, tw_x, tid, nt);
+  line_fft_dispatch<T, -1, false>(n1, plane, gy, tw_y, tid, nt);
+  cx<T>* out = t2 + (col * n2 + zs) * (long)nsticks;
+  for (int s = tid; s < nsticks; s += nt) out[s] = plane[stick_xy[s]];
+}
+
+// C: grid (ceil(nsticks / spb), columns). y = T x + local / N
+template <typename T>
+__global__ __launch_bounds__(256) void k_sticks_forward(const cx<T>* __restrict__ t2, const int* __restrict__ stick_g,
+                                                        int nsticks, int n2, int spb, const cx<T>* __restrict__ tw_z,
+                                                        const double* __restrict__ x, long ldx,
+                                                        const double* __restrict__ kinetic, double* __restrict__ y,
+                                                        long ldy, double inv_count_unused, double count) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lines = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int first = blockIdx.x * spb;
+  const int nl = min(spb, nsticks - first);
+  const long col = blockIdx.y;
+  const cx<T>* in = t2 + col * (long)n2 * nsticks;
+  for (int e = tid; e < nl * n2; e += nt) {
+    const int zs = e / nl, s = e - zs * nl;
+    lines[s * n2 + zs] = in[(long)zs * nsticks + first + s];
+  }
+  __syncthreads();
+  LineGeom geom{nl, n2, 1, false};
+  line_fft_dispatch<T, -1, false>(n2, lines, geom, tw_z, tid, nt);
+  const double* xc = x + 2 * col * ldx;
+  double* yc = y + 2 * col * ldy;
+  for (int e = tid; e < nl * n2; e += nt) {
+    const int s = e / n2, z = e - s * n2;
+    const int g = stick_g[(long)(first + s) * n2 + z];
+    if (g < 0) continue;
+    const double t = kinetic[g];
+    const double2 xi = *reinterpret_cast<const double2*>(xc + 2L * g);
+    const cx<T> w = lines[e];
+    double2 r;
+    r.x = t * xi.x + double(w.x) / count;
+    r.y = t * xi.y + double(w.y) / count;
+    *reinterpret_cast<double2*>(yc + 2L * g) = r;
+  }
+}
+
+// S local part: y = scale * x (used when the operator has no projectors)
+__global__ void k_scale_copy(const double* __restrict__ x, long ldx, double* __restrict__ y, long ldy, long rows,
+                             double scale) {
+  const long col = blockIdx.y;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < 2 * rows; i += (long)gridDim.x * blockDim.x)
+    y[2 * col * ldy + i] = scale * x[2 * col * ldx + i];
+}
+
+// Potential setup: natural-order complex grid -> real slot-order V (FP64 and
+// FP32 copies) plus the largest |Im| / max(1, |Re|) ratio (block maxima).
+__global__ void k_potential_finish(const cx<double>* __restrict__ planes, long count, double* __restrict__ v64,
+                                   float* __restrict__ v32, double* __restrict__ block_ratio) {
+  __shared__ double red[256];
+  double worst = 0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x) {
+    const cx<double> v = planes[i];
+    v64[i] = v.x;
+    v32[i] = float(v.x);
+    worst = fmax(worst, fabs(v.y) / fmax(1.0, fabs(v.x)));
+  }
+  red[threadIdx.x] = worst;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) block_ratio[blockIdx.x] = red[0];
+}
+
+// natural real V(r) -> slot order (drop-in path that receives V(r) directly)
+__global__ void k_potential_to_slots(const double* __restrict__ v_nat, int n0, int n1, int n2, int a0, int a1, int a2,
+                                     double* __restrict__ v64, float* __restrict__ v32) {
+  const long count = (long)n0 * n1 * n2;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x) {
+    // slot layout [zs][xs][ys]
+    const int ys = int(i % n1);
+    const int xs = int((i / n1) % n0);
+    const int zs = int(i / ((long)n0 * n1));
+    const int x = slot_position(xs, a0, n0 / a0);
+    const int y = slot_position(ys, a1, n1 / a1);
+    const int z = slot_position(zs, a2, n2 / a2);
+    const double v = v_nat[((long)x * n1 + y) * n2 + z];
+    v64[i] = v;
+    v32[i] = float(v);
+  }
+}
+
+// slot order -> natural real V(r)
+__global__ void k_potential_to_natural(const double* __restrict__ v64, int n0, int n1, int n2, int a0, int a1, int a2,
+                                       double* __restrict__ v_nat) {
+  const long count = (long)n0 * n1 * n2;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x) {
+    const int ys = int(i % n1);
+    const int xs = int((i / n1) % n0);
+    const int zs = int(i / ((long)n0 * n1));
+    const int x = slot_position(xs, a0, n0 / a0);
+    const int y = slot_position(ys, a1, n1 / a1);
+    const int z = slot_position(zs, a2, n2 / a2);
+    v_nat[((long)x * n1 + y) * n2 + z] = v64[i];
+  }
+}
+
+// ------------------------------------------------------------ host launchers
+namespace {
+int sticks_per_block(int n2) {
+  int spb = 2048 / n2;
+  if (spb < 1) spb = 1;
+  if (spb > 64) spb = 64;
+  return spb;
+}
+}  // namespace
+
+int max_lds_bytes() {
+  static int bytes = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 65536;
+    return v;
+  }();
+  return bytes;
+}
+
+template <typename T>
+hipError_t launch_local_h(const GridPlanDev& p, const double* x, long ldx, long columns, const double* kinetic,
+                          double* y, long ldy, void* t1, void* t2, hipStream_t stream) {
+  if (columns <= 0) return hipSuccess;
+  // one xy-plane per workgroup must fit the LDS of one workgroup
+  if (size_t(p.n0) * p.n1 * sizeof(cx<T>) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
+  const int spb = sticks_per_block(p.n2);
+  const dim3 ga((p.nsticks + spb - 1) / spb, (unsigned)columns);
+  const size_t lds_lines = size_t(spb) * p.n2 * sizeof(cx<T>);
+  const cx<T>* twx = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[0] : p.tw32[0]);
+  const cx<T>* twy = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[1] : p.tw32[1]);
+  const cx<T>* twz = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[2] : p.tw32[2]);
+  const T* v = reinterpret_cast<const T*>(sizeof(T) == 8 ? (const void*)p.v64 : (const void*)p.v32);
+  hipLaunchKernelGGL(k_sticks_backward<T>, ga, dim3(256), lds_lines, stream, x, ldx, p.stick_g, p.nsticks, p.n2, spb,
+                     twz, reinterpret_cast<cx<T>*>(t1));
+  const size_t lds_plane = size_t(p.n0) * p.n1 * sizeof(cx<T>);
+  hipLaunchKernelGGL(k_planes<T>, dim3(p.n2, (unsigned)columns), dim3(256), lds_plane, stream,
+                     reinterpret_cast<const cx<T>*>(t1), reinterpret_cast<cx<T>*>(t2), p.stick_xy, p.nsticks, p.n0,
+                     p.n1, p.n2, twx, twy, v, 0);
+  hipLaunchKernelGGL(k_sticks_forward<T>, ga, dim3(256), lds_lines, stream, reinterpret_cast<const cx<T>*>(t2),
+                     p.stick_g, p.nsticks, p.n2, spb, twz, x, ldx, kinetic, y, ldy, 0.0, double(p.count));
+  return hipGetLastError();
+}
+
+template hipError_t launch_local_h<double>(const GridPlanDev&, const double*, long, long, const double*, double*, long,
+                                           void*, void*, hipStream_t);
+template hipError_t launch_local_h<float>(const GridPlanDev&, const double*, long, long, const double*, double*, long,
+                                          void*, void*, hipStream_t);
+
+hipError_t launch_scale_copy(const double* x, long ldx, double* y, long ldy, long rows, long columns, double scale,
+                             hipStream_t stream) {
+  if (columns <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scale_copy, dim3(64, (unsigned)columns), dim3(256), 0, stream, x, ldx, y, ldy, rows, scale);
+  return hipGetLastError();
+}
+
+// Full backward FFT of a natural complex grid (`grid`, count values) into the
+// slot-ordered real potential. `full` is a plan whose sticks are every (x,y).
+hipError_t launch_potential_setup(const GridPlanDev& full, const double* grid, void* t1, void* t2, double* v64,
+                                  float* v32, double* block_ratio, int ratio_blocks, hipStream_t stream) {
+  if (size_t(full.n0) * full.n1 * sizeof(cx<double>) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
+  const int spb = sticks_per_block(full.n2);
+  const dim3 ga((full.nsticks + spb - 1) / spb, 1);
+  const size_t lds_lines = size_t(spb) * full.n2 * sizeof(cx<double>);
+  hipLaunchKernelGGL(k_sticks_backward<double>, ga, dim3(256), lds_lines, stream, grid, 0L, full.stick_g, full.nsticks,
+                     full.n2, spb, reinterpret_cast<const cx<double>*>(full.tw64[2]), reinterpret_cast<cx<double>*>(t1));
+  const size_t lds_plane = size_t(full.n0) * full.n1 * sizeof(cx<double>);
+  hipLaunchKernelGGL(k_planes<double>, dim3(full.n2, 1), dim3(256), lds_plane, stream,
+                     reinterpret_cast<const cx<double>*>(t1), reinterpret_cast<cx<double>*>(t2), full.stick_xy,
+                     full.nsticks, full.n0, full.n1, full.n2, reinterpret_cast<const cx<double>*>(full.tw64[0]),
+                     reinterpret_cast<const cx<double>*>(full.tw64[1]), (const double*)nullptr, 1);
+  hipLaunchKernelGGL(k_potential_finish, dim3(ratio_blocks), dim3(256), 0, stream,
+                     reinterpret_cast<const cx<double>*>(t2), (long)full.count, v64, v32, block_ratio);
+  return hipGetLastError();
+}
+
+hipError_t launch_potential_to_slots(const GridPlanDev& p, const double* v_nat, double* v64, float* v32,
+                                     hipStream_t stream) {
+  hipLaunchKernelGGL(k_potential_to_slots, dim3(1024), dim3(256), 0, stream, v_nat, p.n0, p.n1, p.n2, p.split[0],
+                     p.split[1], p.split[2], v64, v32);
+  return hipGetLastError();
+}
+
+hipError_t launch_potential_to_natural(const GridPlanDev& p, const double* v64, double* v_nat, hipStream_t stream) {
+  hipLaunchKernelGGL(k_potential_to_natural, dim3(1024), dim3(256), 0, stream, v64, p.n0, p.n1, p.n2, p.split[0],
+                     p.split[1], p.split[2], v_nat);
+  return hipGetLastError();
+}
+
+}  // namespace sdft_hip

@@ -1,0 +1,652 @@
+// Device-resident block Davidson (see davidson.hpp).
+#include "davidson.hpp"
+
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <limits>
+#include <numeric>
+
+#include "../hip/davidson_kernels.hpp"
+
+namespace sdft_hip {
+namespace {
+using C = std::complex<double>;
+double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+class Solver {
+ public:
+  Solver(DeviceOperator& op, const DavidsonParams& p)
+      : op_(op), ctx_(op.context()), s_(ctx_.stream()), p_(p), n_(long(op.n())), R_(p.roots), tol_(p.tolerance) {
+    if (!n_ || R_ <= 0 || R_ > n_ || p.max_iterations <= 0 || !std::isfinite(tol_) || tol_ <= 0)
+      throw std::invalid_argument("HIP: invalid matrix-free Davidson problem");
+    if (p.subspace_factor < 4 || p.subspace_factor > 16)
+      throw std::invalid_argument("HIP: Davidson subspace factor must be in [4,16]");
+    cap_ = std::min<long>(n_, std::max<long>(p.subspace_factor * R_, 64));
+    exact_ = !p.search_fp32;
+    basis_.reserve(2 * n_ * cap_);
+    hb_.reserve(2 * n_ * cap_);
+    sb_.reserve(2 * n_ * cap_);
+    ch_.reserve(2 * cap_ * cap_);
+    cs_.reserve(2 * cap_ * cap_);
+    x_.reserve(2 * n_ * R_);
+    sx_.reserve(2 * n_ * R_);
+    hx_.reserve(2 * n_ * R_);
+    res_.reserve(2 * n_ * R_);
+    // All work buffers are sized once here so that no buffer is reallocated
+    // while kernels that use it are queued.
+    small_.reserve(std::size_t(std::max(16 * cap_, 64 * R_)) + 1024);
+    w1_.reserve(2 * n_ * R_);
+    w2_.reserve(2 * n_ * R_);
+    w3_.reserve(2 * n_ * cap_);
+    w4_.reserve(2 * n_ * cap_);
+    m1_.reserve(8 * R_ * R_ + 2 * cap_ + 64);
+    m2_.reserve(2 * cap_ * cap_);
+    m3_.reserve(4 * cap_ * cap_);
+    rr_.reserve(2 * cap_ * cap_ + 16);
+    rrb_.reserve(2 * cap_ * cap_ + 16);
+    qbuf_.reserve(2 * cap_ * cap_ + 16);
+    corr_.reserve(2 * n_ * R_);
+    retained_.reserve(2 * n_ * cap_);
+  }
+  DavidsonOutput run(const C* initial);
+
+ private:
+  Precision search_prec() const { return exact_ ? Precision::FP64 : Precision::FP32; }
+  Precision prec(bool exact) const { return exact ? Precision::FP64 : search_prec(); }
+  double* col(DevBuf<double>& b, long j, long ld) { return b.p + 2 * j * ld; }
+  void sync() { ctx_.sync(); }
+  void gemm(bool exact, int aop, int bop, long M, long N, long K, double alpha, const double* A, long lda,
+            const double* B, long ldb, double* Cm, long ldc, const double* D = nullptr, long ldd = 0,
+            double beta = 0.0) {
+    const double t0 = p_.timing ? (sync(), now()) : 0;
+    ctx_.zgemm(prec(exact), aop, bop, M, N, K, alpha, A, lda, B, ldb, Cm, ldc, D, ldd, beta);
+    if (p_.timing) { sync(); stats_.seconds_gemm += now() - t0; }
+  }
+  void apply(int which, const double* in, long cols, double* hy, double* sy, bool exact) {
+    const double t0 = p_.timing ? (sync(), now()) : 0;
+    const Precision pr = prec(exact);
+    op_.apply(which, in, cols, hy, sy, pr);
+    if (which != 1) (pr == Precision::FP64 ? stats_.h_columns_exact : stats_.h_columns_search) += cols;
+    if (p_.timing) { sync(); stats_.seconds_apply += now() - t0; }
+  }
+  void copy(double* dst, const double* src, long complex_count) {
+    check(hipMemcpyAsync(dst, src, 16 * std::size_t(complex_count), hipMemcpyDeviceToDevice, s_), "copy");
+  }
+  std::vector<double> to_host(const double* d, long count) {
+    std::vector<double> h(count);
+    check(hipMemcpyAsync(h.data(), d, 8 * std::size_t(count), hipMemcpyDeviceToHost, s_), "to host");
+    sync();
+    return h;
+  }
+  // small host -> device transfers of control scalars; synchronous so that
+  // the host source may be reused immediately
+  void upload(double* d, const double* h, long count) {
+    check(hipMemcpyAsync(d, h, 8 * std::size_t(count), hipMemcpyHostToDevice, s_), "upload");
+    sync();
+  }
+  std::vector<double> col_norms(const double* x, long ld, long cols) {
+    double* o = small_.p;
+    check(launch_col_norms(x, ld, n_, cols, o, s_), "norms");
+    return to_host(o, cols);
+  }
+  std::vector<double> real_dots(const double* a, const double* b, long cols) {
+    double* o = small_.p;
+    check(launch_real_dots(a, n_, b, n_, n_, cols, o, s_), "dots");
+    return to_host(o, cols);
+  }
+  void check_finite(const double* x, long cols, const char* what) {
+    int* flag = reinterpret_cast<int*>(small_.p);
+    check(hipMemsetAsync(flag, 0, sizeof(int), s_), "memset");
+    check(launch_finite_check(x, n_, n_, cols, flag, s_), "finite");
+    int bad = 0;
+    check(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, s_), "finite copy");
+    sync();
+    if (bad) throw std::runtime_error(std::string("HIP: nonfinite ") + what);
+  }
+  // Hermitian eigen-decomposition of a device matrix (overwritten by vectors)
+  std::vector<double> heevd(double* a, long m, long lda) {
+    double* w = small_.p + 8;
+    double* e = w + m;
+    int* info = reinterpret_cast<int*>(small_.p);
+    const double t0 = p_.timing ? (sync(), now()) : 0;
+    check(rocsolver_zheevd(ctx_.blas(), rocblas_evect_original, rocblas_fill_lower, rocblas_int(m),
+                           reinterpret_cast<rocblas_double_complex*>(a), rocblas_int(lda), w, e, info),
+          "rocsolver_zheevd");
+    std::vector<double> h = to_host(small_.p, 8 + m);
+    if (p_.timing) stats_.seconds_ritz += now() - t0;
+    if (*reinterpret_cast<int*>(h.data())) throw std::runtime_error("HIP: Hermitian eigensolve failed");
+    return std::vector<double>(h.begin() + 8, h.begin() + 8 + m);
+  }
+  // generalized A v = l B v (itype 1), A overwritten by B-orthonormal vectors
+  std::vector<double> hegvd(double* a, double* b, long m, long ld) {
+    double* w = small_.p + 8;
+    double* e = w + m;
+    int* info = reinterpret_cast<int*>(small_.p);
+    const double t0 = p_.timing ? (sync(), now()) : 0;
+    check(rocsolver_zhegvd(ctx_.blas(), rocblas_eform_ax, rocblas_evect_original, rocblas_fill_lower,
+                           rocblas_int(m), reinterpret_cast<rocblas_double_complex*>(a), rocblas_int(ld),
+                           reinterpret_cast<rocblas_double_complex*>(b), rocblas_int(ld), w, e, info),
+          "rocsolver_zhegvd");
+    std::vector<double> h = to_host(small_.p, 8 + m);
+    if (p_.timing) stats_.seconds_ritz += now() - t0;
+    if (*reinterpret_cast<int*>(h.data())) throw std::runtime_error("HIP: Davidson Rayleigh-Ritz failed");
+    return std::vector<double>(h.begin() + 8, h.begin() + 8 + m);
+  }
+
+  bool append(const double* v_in);
+  bool append_block(double* v, long cols);
+  void update_projected(const double* applied, double* cached);
+  void flush_h();
+  void certify();
+  void refresh_norms() {
+    res_norms_ = col_norms(res_.p, n_, R_);
+    converged_ = true;
+    for (long r = 0; r < R_; ++r) converged_ = converged_ && res_norms_[r] <= tol_;
+  }
+  double max_res() const { return *std::max_element(res_norms_.begin(), res_norms_.end()); }
+  void rebuild_exact() {  // fresh FP64 pair images of the whole basis
+    apply(2, basis_.p, size_, hb_.p, sb_.p, true);
+    computed_h_ = size_;
+    projected_size_ = 0;
+  }
+  bool polish_step(double* direction);
+  bool manifold_step();
+  void preserved_subspace(long keep, double* retained);
+
+  DeviceOperator& op_;
+  Context& ctx_;
+  hipStream_t s_;
+  DavidsonParams p_;
+  long n_, R_, cap_ = 0;
+  double tol_;
+  bool exact_ = true, incremental_ = true, converged_ = false;
+  DevBuf<double> basis_, hb_, sb_, ch_, cs_, x_, sx_, hx_, res_;
+  DevBuf<double> small_, w1_, w2_, w3_, w4_, m1_, m2_, m3_, rr_, rrb_, qbuf_, corr_, retained_;
+  long size_ = 0, computed_h_ = 0, projected_size_ = 0;
+  std::vector<double> values_, res_norms_;
+  DavidsonStats stats_;
+};
+
+// src/pw_core.cpp:796-853: serial CGS2 of one direction in exact FP64
+bool Solver::append(const double* v_in) {
+  if (size_ >= cap_) return false;
+  double* v = w1_.p;
+  double* sv = w2_.p;
+  copy(v, v_in, n_);
+  const double en = col_norms(v, n_, 1)[0];
+  if (!std::isfinite(en)) throw std::runtime_error("HIP: nonfinite Davidson direction");
+  if (en == 0) return false;
+  upload(small_.p, &en, 1);
+  check(launch_divide_columns(v, n_, n_, 1, small_.p, s_), "scale");
+  apply(1, v, 1, nullptr, sv, true);
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass) apply(1, v, 1, nullptr, sv, true);
+    if (size_ < 8) {
+      for (long j = 0; j < size_; ++j) {
+        double* proj = m1_.p;
+        gemm(true, 1, 0, 1, 1, n_, 1.0, col(basis_, j, n_), n_, sv, n_, proj, 1);
+        gemm(true, 0, 0, n_, 1, 1, -1.0, col(basis_, j, n_), n_, proj, 1, v, n_, v, n_, 1.0);
+        gemm(true, 0, 0, n_, 1, 1, -1.0, col(sb_, j, n_), n_, proj, 1, sv, n_, sv, n_, 1.0);
+      }
+    } else {
+      double* proj = m1_.p;
+      gemm(true, 1, 0, size_, 1, n_, 1.0, basis_.p, n_, sv, n_, proj, size_);
+      gemm(true, 0, 0, n_, 1, size_, -1.0, basis_.p, n_, proj, size_, v, n_, v, n_, 1.0);
+    }
+  }
+  apply(1, v, 1, nullptr, sv, true);
+  double* nn = m1_.p;
+  gemm(true, 1, 0, 1, 1, n_, 1.0, v, n_, sv, n_, nn, 1);
+  const double norm2 = to_host(nn, 1)[0];
+  if (!std::isfinite(norm2) || norm2 < -1e-10) throw std::runtime_error("HIP: Davidson encountered non-positive overlap direction");
+  if (norm2 < 1e-24) return false;
+  const double r = std::sqrt(norm2);
+  upload(small_.p, &r, 1);
+  check(launch_divide_columns(v, n_, n_, 1, small_.p, s_), "scale");
+  check(launch_divide_columns(sv, n_, n_, 1, small_.p, s_), "scale");
+  copy(col(basis_, size_, n_), v, n_);
+  copy(col(sb_, size_, n_), sv, n_);
+  ++size_;
+  return true;
+}
+
+// src/pw_core.cpp:871-1013: block CGS2 in the S metric with eigen-normalisation.
+// `v` (n x cols, ld n) is a scratch block owned by the caller.
+bool Solver::append_block(double* v, long cols) {
+  const long before = size_;
+  cols = std::min(cols, cap_ - size_);
+  if (cols <= 0) return false;
+  auto serial = [&]() {
+    for (long j = 0; j < cols; ++j) append(v + 2 * j * n_);
+    return size_ > before;
+  };
+  if (cols == 1) return serial();
+  const double t0 = p_.timing ? (sync(), now()) : 0;
+  const bool exact = exact_;  // exact_orthogonalize: search arithmetic before the hand-off
+  const std::vector<double> len = col_norms(v, n_, cols);
+  for (double l : len)
+    if (!(l > 0) || !std::isfinite(l)) return serial();
+  upload(small_.p + 8, len.data(), cols);
+  check(launch_divide_columns(v, n_, n_, cols, small_.p + 8, s_), "normalise");
+  double* original = w3_.p;
+  double* metric = w4_.p;
+  copy(original, v, n_ * cols);
+  apply(1, v, cols, nullptr, metric, exact);
+  for (int pass = 0; size_ && pass < 2; ++pass) {
+    double* proj = m2_.p;
+    gemm(exact, 1, 0, size_, cols, n_, 1.0, basis_.p, n_, metric, n_, proj, size_);
+    gemm(exact, 0, 0, n_, cols, size_, -1.0, basis_.p, n_, proj, size_, v, n_, v, n_, 1.0);
+    apply(1, v, cols, nullptr, metric, exact);
+  }
+  double* gram = m3_.p;
+  double* gh = m3_.p + 2 * cols * cols;
+  gemm(exact, 1, 0, cols, cols, n_, 1.0, v, n_, metric, n_, gram, cols);
+  check(launch_hermitise(gram, cols, cols, gh, cols, s_), "hermitise");
+  const std::vector<double> lam = heevd(gh, cols, cols);
+  const double largest = *std::max_element(lam.begin(), lam.end());
+  const double smallest = *std::min_element(lam.begin(), lam.end());
+  if (smallest < -1e-10) throw std::runtime_error("HIP: block Davidson encountered non-positive overlap");
+  if (!(largest > 0) || smallest < 1e-10 * largest) {
+    copy(v, original, n_ * cols);
+    if (p_.timing) stats_.seconds_ortho += now() - t0;
+    return serial();
+  }
+  upload(small_.p + 8, lam.data(), cols);
+  check(launch_inv_sqrt_columns(gh, cols, cols, cols, small_.p + 8, s_), "normalisation");
+  gemm(exact, 0, 0, n_, cols, cols, 1.0, v, n_, gh, cols, col(basis_, size_, n_), n_);
+  gemm(exact, 0, 0, n_, cols, cols, 1.0, metric, n_, gh, cols, col(sb_, size_, n_), n_);
+  size_ += cols;
+  if (p_.timing) { sync(); stats_.seconds_ortho += now() - t0; }
+  return true;
+}
+
+// src/pw_core.cpp:771-785
+void Solver::update_projected(const double* applied, double* cached) {
+  const long first = incremental_ ? projected_size_ : 0;
+  if (first == size_) return;
+  const long cols = size_ - first;
+  gemm(exact_, 1, 0, size_, cols, n_, 1.0, basis_.p, n_, applied + 2 * first * n_, n_, cached + 2 * first * cap_,
+       cap_);
+  if (first)
+    gemm(exact_, 1, 0, cols, first, n_, 1.0, basis_.p + 2 * first * n_, n_, applied, n_, cached + 2 * first, cap_);
+}
+
+void Solver::flush_h() {
+  if (computed_h_ >= size_) return;
+  apply(0, col(basis_, computed_h_, n_), size_ - computed_h_, col(hb_, computed_h_, n_), nullptr, exact_);
+  computed_h_ = size_;
+}
+
+// src/pw_core.cpp:1125-1149 (certify_once = false)
+void Solver::certify() {
+  apply(1, x_.p, R_, nullptr, sx_.p, true);
+  const std::vector<double> norm2 = real_dots(x_.p, sx_.p, R_);
+  std::vector<double> root(R_);
+  for (long r = 0; r < R_; ++r) {
+    if (!std::isfinite(norm2[r]) || !(norm2[r] > 0)) throw std::runtime_error("HIP: invalid Davidson Ritz S norm");
+    root[r] = std::sqrt(norm2[r]);
+  }
+  upload(small_.p + 8, root.data(), R_);
+  check(launch_divide_columns(x_.p, n_, n_, R_, small_.p + 8, s_), "normalise");
+  apply(2, x_.p, R_, hx_.p, sx_.p, true);
+  const std::vector<double> metric = real_dots(x_.p, sx_.p, R_);
+  const std::vector<double> energy = real_dots(x_.p, hx_.p, R_);
+  for (long r = 0; r < R_; ++r) {
+    if (!(metric[r] > 0)) throw std::runtime_error("HIP: invalid Davidson Rayleigh metric");
+    values_[r] = energy[r] / metric[r];
+  }
+  upload(small_.p + 8, values_.data(), R_);
+  check(launch_fma_residual(hx_.p, sx_.p, n_, n_, R_, small_.p + 8, res_.p, s_), "residual");
+  refresh_norms();
+}
+
+// src/strict_residual_polish.hpp:125-197 (no locked bands)
+bool Solver::polish_step(double* direction) {
+  const double old_max = max_res();
+  if (old_max <= tol_) return false;
+  for (int pass = 0; pass < 2; ++pass) {
+    gemm(true, 1, 0, R_, R_, n_, 1.0, sx_.p, n_, direction, n_, m1_.p, R_);
+    gemm(true, 0, 0, n_, R_, R_, -1.0, x_.p, n_, m1_.p, R_, direction, n_, direction, n_, 1.0);
+  }
+  apply(2, direction, R_, w1_.p, w2_.p, true);  // dh, ds
+  const std::vector<double> grad = real_dots(direction, res_.p, R_);
+  const std::vector<double> dhd = real_dots(direction, w1_.p, R_);
+  const std::vector<double> dsd = real_dots(direction, w2_.p, R_);
+  const std::vector<double> dn = col_norms(direction, n_, R_);
+  std::vector<double> alpha(R_, 0.0);
+  bool moved = false;
+  for (long r = 0; r < R_; ++r) {
+    if (res_norms_[r] <= .5 * tol_) continue;
+    const double curvature = dhd[r] - values_[r] * dsd[r];
+    if (!(curvature > 0.) || !std::isfinite(curvature) || !std::isfinite(grad[r])) continue;
+    const double a = -grad[r] / curvature;
+    if (!(a > 0.) || !std::isfinite(a) || a * dn[r] > 1e-4) continue;
+    alpha[r] = a;
+    moved = true;
+  }
+  if (!moved) return false;
+  double* trial = w3_.p;
+  upload(small_.p + 8, alpha.data(), R_);
+  check(launch_polish_update(x_.p, direction, n_, n_, R_, small_.p + 8, trial, s_), "polish update");
+  apply(2, trial, R_, w1_.p, w2_.p, true);
+  const std::vector<double> metric = real_dots(trial, w2_.p, R_);
+  std::vector<double> root(R_);
+  for (long r = 0; r < R_; ++r) {
+    if (!(metric[r] > 0.) || !std::isfinite(metric[r])) return false;
+    root[r] = std::sqrt(metric[r]);
+  }
+  upload(small_.p + 8, root.data(), R_);
+  check(launch_divide_columns(trial, n_, n_, R_, small_.p + 8, s_), "normalise");
+  apply(2, trial, R_, w1_.p, w2_.p, true);
+  const std::vector<double> m2 = real_dots(trial, w2_.p, R_);
+  const std::vector<double> e2 = real_dots(trial, w1_.p, R_);
+  std::vector<double> energies(R_);
+  for (long r = 0; r < R_; ++r) energies[r] = e2[r] / m2[r];
+  upload(small_.p + 8, energies.data(), R_);
+  double* errors = w4_.p;
+  check(launch_fma_residual(w1_.p, w2_.p, n_, n_, R_, small_.p + 8, errors, s_), "errors");
+  const std::vector<double> en = col_norms(errors, n_, R_);
+  const double new_max = *std::max_element(en.begin(), en.end());
+  gemm(true, 1, 0, R_, R_, n_, 1.0, trial, n_, w2_.p, n_, m1_.p, R_);
+  const std::vector<double> g = to_host(m1_.p, 2 * R_ * R_);
+  double gerr = 0;
+  for (long j = 0; j < R_; ++j)
+    for (long i = 0; i < R_; ++i)
+      gerr = std::max(gerr, std::abs(C(g[2 * (i + j * R_)], g[2 * (i + j * R_) + 1]) - (i == j ? C(1) : C(0))));
+  for (double e : en)
+    if (!std::isfinite(e)) return false;
+  if (!(gerr <= 1e-12) || !(new_max <= tol_ || new_max < .999 * old_max)) return false;
+  copy(x_.p, trial, n_ * R_);
+  copy(sx_.p, w2_.p, n_ * R_);
+  copy(res_.p, errors, n_ * R_);
+  values_ = energies;
+  return true;
+}
+
+// src/strict_residual_polish.hpp:68-123 (no locked bands)
+bool Solver::manifold_step() {
+  const double old_max = max_res();
+  if (old_max <= tol_) return false;
+  apply(2, x_.p, R_, w1_.p, w2_.p, true);
+  double* h = m1_.p;
+  double* s = m1_.p + 2 * R_ * R_;
+  double* hh = m1_.p + 4 * R_ * R_;
+  double* sh = m1_.p + 6 * R_ * R_;
+  check(launch_complex_dots(x_.p, n_, R_, w1_.p, n_, R_, n_, h, R_, s_), "manifold h");
+  check(launch_complex_dots(x_.p, n_, R_, w2_.p, n_, R_, n_, s, R_, s_), "manifold s");
+  check(launch_hermitise(h, R_, R_, hh, R_, s_), "hermitise");
+  check(launch_hermitise(s, R_, R_, sh, R_, s_), "hermitise");
+  hegvd(hh, sh, R_, R_);
+  double* trial = w3_.p;
+  gemm(true, 0, 0, n_, R_, R_, 1.0, x_.p, n_, hh, R_, trial, n_);
+  apply(2, trial, R_, w1_.p, w2_.p, true);
+  const std::vector<double> norm = real_dots(trial, w2_.p, R_);
+  std::vector<double> root(R_);
+  for (long r = 0; r < R_; ++r) {
+    if (!std::isfinite(norm[r]) || norm[r] <= 0) return false;
+    root[r] = std::sqrt(norm[r]);
+  }
+  upload(small_.p + 8, root.data(), R_);
+  check(launch_divide_columns(trial, n_, n_, R_, small_.p + 8, s_), "normalise");
+  apply(2, trial, R_, w1_.p, w2_.p, true);
+  const std::vector<double> e1 = real_dots(trial, w1_.p, R_);
+  const std::vector<double> m = real_dots(trial, w2_.p, R_);
+  std::vector<double> energies(R_);
+  for (long r = 0; r < R_; ++r) energies[r] = e1[r] / m[r];
+  upload(small_.p + 8, energies.data(), R_);
+  double* errors = w4_.p;
+  check(launch_fma_residual(w1_.p, w2_.p, n_, n_, R_, small_.p + 8, errors, s_), "errors");
+  const std::vector<double> en = col_norms(errors, n_, R_);
+  const double new_max = *std::max_element(en.begin(), en.end());
+  gemm(true, 1, 0, R_, R_, n_, 1.0, trial, n_, w2_.p, n_, h, R_);
+  const std::vector<double> g = to_host(h, 2 * R_ * R_);
+  double gerr = 0;
+  for (long j = 0; j < R_; ++j)
+    for (long i = 0; i < R_; ++i)
+      gerr = std::max(gerr, std::abs(C(g[2 * (i + j * R_)], g[2 * (i + j * R_) + 1]) - (i == j ? C(1) : C(0))));
+  if (!std::isfinite(new_max) || gerr > 1e-12 || !(new_max <= tol_ || new_max < .999 * old_max)) return false;
+  copy(x_.p, trial, n_ * R_);
+  copy(sx_.p, w2_.p, n_ * R_);
+  copy(res_.p, errors, n_ * R_);
+  values_ = energies;
+  return true;
+}
+
+// src/strict_residual_polish.hpp:47-63: [x, B Q_perp] with Q from a Householder
+// QR of B^H sx (size x roots).
+void Solver::preserved_subspace(long keep, double* retained) {
+  copy(retained, x_.p, n_ * R_);
+  if (keep <= R_) return;
+  const long m = size_;
+  double* q = qbuf_.p;
+  gemm(true, 1, 0, m, R_, n_, 1.0, basis_.p, n_, sx_.p, n_, q, m);
+  auto* tau = reinterpret_cast<rocblas_double_complex*>(small_.p + 8);
+  check(rocsolver_zgeqrf(ctx_.blas(), rocblas_int(m), rocblas_int(R_), reinterpret_cast<rocblas_double_complex*>(q),
+                         rocblas_int(m), tau),
+        "rocsolver_zgeqrf");
+  check(rocsolver_zungqr(ctx_.blas(), rocblas_int(m), rocblas_int(m), rocblas_int(R_),
+                         reinterpret_cast<rocblas_double_complex*>(q), rocblas_int(m), tau),
+        "rocsolver_zungqr");
+  gemm(true, 0, 0, n_, keep - R_, m, 1.0, basis_.p, n_, q + 2 * R_ * m, m, retained + 2 * R_ * n_, n_);
+}
+
+DavidsonOutput Solver::run(const C* initial) {
+  const double t_start = now();
+  const long n = n_, R = R_;
+  incremental_ = tol_ >= 1e-12;
+  bool strict_projection = false;
+  unsigned polish_attempts = 0, manifold_attempts = 0;
+  double strict_entry = std::numeric_limits<double>::infinity();
+  const std::vector<double>& hd = op_.h_diagonal();
+  const std::vector<double>& sd = op_.s_diagonal();
+  std::vector<long> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](long i, long j) { return hd[i] / sd[i] < hd[j] / sd[j]; });
+  // seeds (src/pw_core.cpp:1027-1112): splitmix64 broad seeds, warm start +
+  // clamp(100 tol, 1e-10, 1e-6) perturbation, 2 roots canonical guards
+  std::uint64_t state = 0x6a09e667f3bcc909ULL;
+  auto uniform = [&]() {
+    std::uint64_t z = (state += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    z ^= z >> 31;
+    return static_cast<double>(z >> 11) * 0x1.0p-53 - .5;
+  };
+  auto broad = [&](std::vector<C>& s) {
+    s.resize(n);
+    double nn = 0;
+    for (long i = 0; i < n; ++i) {
+      const double re = uniform();
+      const double im = uniform();
+      s[i] = C(re, im);
+      nn += std::norm(s[i]);
+    }
+    nn = std::sqrt(nn);
+    for (auto& z : s) z /= nn;
+  };
+  const double warm_pert = std::clamp(100. * tol_, 1e-10, 1e-6);
+  const bool warm = initial != nullptr && 2 * R < n;
+  const long guards = std::min(n, 2 * R);
+  const long nseeds = (warm ? R : 0) + guards;
+  std::vector<C> seeds(std::size_t(n) * nseeds);
+  std::vector<C> b;
+  long sc = 0;
+  if (warm)
+    for (long j = 0; j < R; ++j, ++sc) {
+      const C* src = initial + std::size_t(j) * n;
+      double nn = 0;
+      for (long i = 0; i < n; ++i) nn += std::norm(src[i]);
+      nn = std::sqrt(nn);
+      broad(b);
+      for (long i = 0; i < n; ++i) seeds[std::size_t(sc) * n + i] = src[i] + warm_pert * nn * b[i];
+    }
+  for (long j = 0; j < guards; ++j, ++sc) {
+    broad(b);
+    for (long i = 0; i < n; ++i) seeds[std::size_t(sc) * n + i] = .05 * b[i];
+    seeds[std::size_t(sc) * n + order[j]] += 1.;
+  }
+  DevBuf<double> block;
+  block.upload(reinterpret_cast<const double*>(seeds.data()), 2 * seeds.size(), s_);
+  append_block(block.p, nseeds);
+
+  values_.assign(R, 0.);
+  res_norms_.assign(R, 0.);
+  bool certified = false;
+  long iterations = 0;
+  DevBuf<double>& rrb = rrb_;
+  DevBuf<double>& corr = corr_;
+  DevBuf<double>& retained = retained_;
+  double best_history = std::numeric_limits<double>::infinity();
+  long since_halving = 0;
+  for (long it = 0; it < p_.max_iterations; ++it) {
+    flush_h();
+    update_projected(hb_.p, ch_.p);
+    update_projected(sb_.p, cs_.p);
+    projected_size_ = size_;
+    double* A = rr_.p;  // eigenvectors after the solve
+    double* B = rrb.p;
+    check(launch_hermitise(ch_.p, cap_, size_, A, size_, s_), "hermitise");
+    check(launch_hermitise(cs_.p, cap_, size_, B, size_, s_), "hermitise");
+    const std::vector<double> w = hegvd(A, B, size_, size_);
+    for (long r = 0; r < R; ++r) values_[r] = w[r];
+    // Ritz vectors and residuals (src/pw_core.cpp:1169-1183)
+    gemm(exact_, 0, 0, n, R, size_, 1.0, basis_.p, n, A, size_, x_.p, n);
+    gemm(exact_, 0, 0, n, R, size_, 1.0, sb_.p, n, A, size_, sx_.p, n);
+    gemm(exact_, 0, 0, n, R, size_, 1.0, hb_.p, n, A, size_, hx_.p, n);
+    upload(small_.p + 8, values_.data(), R);
+    check(launch_ritz_residual(hx_.p, sx_.p, n, n, R, small_.p + 8, res_.p, s_), "ritz residual");
+    refresh_norms();
+    iterations = it + 1;
+    stats_.max_residual_history.push_back(max_res());
+    if (p_.verbose)
+      std::fprintf(stderr, "davidson it=%ld size=%ld exact=%d max_residual=%.3e\n", it + 1, size_, int(exact_), max_res());
+    if (!exact_) {
+      // FP32 search hand-off: the reference's 1e-8 threshold, or a stalled
+      // FP32 residual (its rounding floor lies far above 1e-8)
+      if (max_res() < 0.5 * best_history) { best_history = max_res(); since_halving = 0; } else ++since_halving;
+      const bool stalled = since_halving >= p_.stagnation_window;
+      if (max_res() < p_.search_handoff || it + 1 == p_.max_iterations || stalled) {
+        exact_ = true;
+        stats_.handoff_iteration = it + 1;
+        rebuild_exact();
+        continue;
+      }
+    }
+    bool fresh = false;
+    if (max_res() <= 8 * tol_) { certify(); fresh = true; }
+    if (converged_ && !fresh) { certify(); fresh = true; }
+    if (!converged_ && tol_ <= 1e-12 && !strict_projection && max_res() <= 1e-10) {
+      if (!fresh) { certify(); fresh = true; }
+      if (!converged_) {  // src/pw_core.cpp:1247-1267
+        rebuild_exact();
+        incremental_ = false;
+        strict_projection = true;
+        continue;
+      }
+    }
+    bool polished = false;
+    if (!converged_ && tol_ <= 1e-12 && max_res() <= 1e-10 && max_res() < .5 * strict_entry &&
+        (polish_attempts < 16 || manifold_attempts < 4)) {
+      if (!fresh) certify();
+      strict_entry = max_res();
+      if (!converged_ && manifold_attempts < 4 && it + 1 < p_.max_iterations) {
+        ++manifold_attempts;
+        ++it;
+        iterations = it + 1;
+        polished |= manifold_step();
+        refresh_norms();
+      }
+      for (; !converged_ && polish_attempts < 16 && it + 1 < p_.max_iterations; ++polish_attempts) {
+        std::vector<int> src(R);
+        std::iota(src.begin(), src.end(), 0);
+        check(launch_kinetic_weights(x_.p, n, n, R, op_.kinetic_device(), small_.p + 8, s_), "kinetic weights");
+        int* srcd = reinterpret_cast<int*>(small_.p + 8 + R);
+        check(hipMemcpyAsync(srcd, src.data(), R * sizeof(int), hipMemcpyHostToDevice, s_), "src");
+        check(launch_precondition(res_.p, n, srcd, R, small_.p + 8, op_.kinetic_device(), n, corr.p, n, s_),
+              "precondition");
+        ++it;
+        iterations = it + 1;
+        const bool improved = polish_step(corr.p);
+        polished |= improved;
+        if (!improved) { ++polish_attempts; break; }
+        refresh_norms();
+      }
+    }
+    if (converged_) { certified = true; break; }
+    if (polished && it + 1 < p_.max_iterations) {
+      const bool at_cap = size_ + R > cap_;
+      const long keep = at_cap ? std::min(size_, 2 * R) : size_;
+      if (at_cap) {
+        copy(retained.p, x_.p, n * R);
+        if (keep > R) gemm(exact_, 0, 0, n, keep - R, size_, 1.0, basis_.p, n, A + 2 * R * size_, size_, retained.p + 2 * R * n, n);
+      } else {
+        preserved_subspace(keep, retained.p);
+      }
+      size_ = 0;
+      computed_h_ = 0;
+      projected_size_ = 0;
+      append_block(retained.p, keep);
+    }
+    if (size_ + R > cap_) {  // thick restart (src/pw_core.cpp:1357-1383)
+      const long keep = std::min(size_, 2 * R);
+      gemm(exact_, 0, 0, n, keep, size_, 1.0, basis_.p, n, A, size_, retained.p, n);
+      size_ = 0;
+      computed_h_ = 0;
+      projected_size_ = 0;
+      append_block(retained.p, keep);
+    }
+    const long previous = size_;
+    std::vector<int> src;
+    for (long r = 0; r < R && size_ + long(src.size()) < cap_; ++r)
+      if (res_norms_[r] > .5 * tol_) src.push_back(int(r));
+    // the reference stops collecting once size reaches capacity; it appends
+    // at most cap - size columns (append_block truncates the same way)
+    if (!src.empty()) {
+      check(launch_kinetic_weights(x_.p, n, n, R, op_.kinetic_device(), small_.p + 8, s_), "kinetic weights");
+      int* srcd = reinterpret_cast<int*>(small_.p + 8 + R);
+      check(hipMemcpyAsync(srcd, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice, s_), "src");
+      check(launch_precondition(res_.p, n, srcd, long(src.size()), small_.p + 8, op_.kinetic_device(), n, corr.p, n, s_),
+            "precondition");
+      append_block(corr.p, long(src.size()));
+    }
+    if (size_ == previous) {
+      bool added = false;
+      std::vector<double> unit(2 * n, 0.0);
+      DevBuf<double> u;
+      u.reserve(2 * n);
+      for (long j = 0; j < n && !added; ++j) {
+        std::fill(unit.begin(), unit.end(), 0.0);
+        unit[2 * order[j]] = 1.0;
+        upload(u.p, unit.data(), 2 * n);
+        added = append(u.p);
+      }
+      if (!added) break;
+    }
+  }
+  if (!certified) certify();
+  check_finite(x_.p, R, "Davidson eigenvectors");
+  DavidsonOutput out;
+  out.eigenvalues = values_;
+  out.residual_norms = res_norms_;
+  out.eigenvectors.resize(std::size_t(n) * R);
+  check(hipMemcpyAsync(out.eigenvectors.data(), x_.p, 16 * std::size_t(n) * R, hipMemcpyDeviceToHost, s_), "vectors");
+  sync();
+  stats_.iterations = iterations;
+  stats_.converged = converged_;
+  stats_.seconds_total = now() - t_start;
+  out.stats = stats_;
+  return out;
+}
+}  // namespace
+
+DavidsonOutput davidson(DeviceOperator& op, const DavidsonParams& params, const std::complex<double>* initial) {
+  Solver s(op, params);
+  return s.run(initial);
+}
+
+}  // namespace sdft_hip

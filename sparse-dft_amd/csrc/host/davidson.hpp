@@ -1,0 +1,45 @@
+// Device-resident block Davidson in the S metric, restating
+// solve_generalized_hermitian_davidson (src/pw_core.cpp:532-1437) with the
+// reference's default controls, strict polish and root-manifold refinement
+// (src/strict_residual_polish.hpp:47-197). Every plane-wave-sized block stays
+// in HBM; only residual norms, Ritz values and small control scalars cross to
+// the host.
+#pragma once
+#include <complex>
+#include <vector>
+
+#include "device.hpp"
+
+namespace sdft_hip {
+
+struct DavidsonParams {
+  long roots = 0;
+  long max_iterations = 256;
+  double tolerance = 1e-12;
+  long subspace_factor = 8;       // capacity = min(n, max(factor*roots, 64)) (src/pw_core.cpp:735-746)
+  int search_fp32 = 0;            // 1: FP32 H/S applies and subspace GEMMs until hand-off
+  double search_handoff = 1e-8;   // max residual that hands the search to FP64 (src/pw_core.cpp:1184)
+  int stagnation_window = 3;      // FP32 only: hand off when max residual stops halving over this many iterations
+  int verbose = 0;
+  int timing = 0;                 // synchronise phases and accumulate per-phase times
+};
+
+struct DavidsonStats {
+  long iterations = 0;
+  bool converged = false;
+  long h_columns_search = 0, h_columns_exact = 0;
+  long handoff_iteration = -1;
+  double seconds_total = 0, seconds_apply = 0, seconds_gemm = 0, seconds_ritz = 0, seconds_ortho = 0;
+  std::vector<double> max_residual_history;
+};
+
+struct DavidsonOutput {
+  std::vector<double> eigenvalues, residual_norms;
+  std::vector<std::complex<double>> eigenvectors;  // column-major n x roots
+  DavidsonStats stats;
+};
+
+DavidsonOutput davidson(DeviceOperator& op, const DavidsonParams& params,
+                        const std::complex<double>* initial /* n x roots or nullptr */);
+
+}  // namespace sdft_hip

@@ -1,0 +1,193 @@
+// Device context and the device-resident plane-wave operator (the HIP-side
+// state behind the C ABI). One Context per host thread, as the reference keeps
+// one thread_local Accelerator per thread (src/compute_backend.cpp:62-73).
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <rocblas/rocblas.h>
+
+#include <array>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../hip/gemm.hpp"
+#include "../hip/local_h.hpp"
+
+namespace sdft_hip {
+
+void check(hipError_t e, const char* what);
+void check(rocblas_status s, const char* what);
+
+// Grow-only device buffer (src/compute_cuda.cu:72-79 reserve semantics).
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  std::size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  T* reserve(std::size_t count) {
+    if (count > n) {
+      release();
+      check(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
+      n = count;
+    }
+    return p;
+  }
+  void upload(const T* host, std::size_t count, hipStream_t s) {
+    reserve(count);
+    if (count) check(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+  }
+};
+
+struct PerfCounter {
+  double seconds = 0;
+  std::size_t calls = 0;
+};
+
+// Host description of the sphere-restricted grid plan.
+struct HostGridPlan {
+  std::array<int, 3> shape{};
+  std::array<int, 3> split{};
+  std::vector<int> stick_xy;  // nsticks
+  std::vector<int> stick_g;   // nsticks * n2
+  long count = 0;
+  int nsticks() const { return int(stick_xy.size()); }
+};
+HostGridPlan build_grid_plan(std::array<int, 3> shape, const std::uint64_t* indices, std::size_t n);
+HostGridPlan build_full_plan(std::array<int, 3> shape);
+
+// Device copy of a grid plan with twiddles (potential buffers owned separately).
+struct DeviceGrid {
+  GridPlanDev dev;
+  DevBuf<int> stick_g, stick_xy;
+  DevBuf<double> tw64[3];
+  DevBuf<float> tw32[3];
+  void upload(const HostGridPlan& h, hipStream_t s);
+};
+
+class Context {
+ public:
+  explicit Context(int device);
+  ~Context();
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  rocblas_handle blas() const { return blas_; }
+  const std::string& name() const { return name_; }
+  void sync() { check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+  double* gemm_work(std::size_t bytes) { return work_.reserve((bytes + 15) / 16 * 2); }
+  std::size_t gemm_work_bytes() const { return work_.n * sizeof(double); }
+  // C = alpha op(A) op(B) + beta D on device pointers, workspace managed here.
+  void zgemm(Precision prec, int aop, int bop, long M, long N, long K, double alpha, const double* A, long lda,
+             const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta);
+  void record(const std::string& name, double seconds, std::size_t calls = 1);
+  std::map<std::string, PerfCounter> counters;
+
+  // caches for the drop-in host-pointer entry points (identity tokens)
+  struct Resident {
+    std::uint64_t identity = 0;
+    std::size_t rows = 0, cols = 0, ld = 0;
+    DevBuf<double> values;
+  } resident_left[2];
+  DevBuf<double> ga, gb, gc;
+  struct LocalCache {
+    std::uint64_t identity = 0;
+    std::array<int, 3> shape{};
+    std::size_t npw = 0;
+    DeviceGrid grid;
+    DevBuf<double> v64, kinetic, vnat;
+    DevBuf<float> v32;
+    DevBuf<char> t1, t2;
+    DevBuf<double> in, out;
+  } local;
+  struct PawCache {
+    std::uint64_t identity = 0, projector_identity = 0;
+    std::size_t npw = 0, np = 0;
+    DevBuf<double> b, d, q, in, p, dq, h, s;
+  } paw;
+
+ private:
+  int device_;
+  hipStream_t stream_{};
+  rocblas_handle blas_{};
+  std::string name_;
+  DevBuf<double> work_;
+};
+
+struct OperatorDesc {
+  std::size_t n = 0;
+  const int* hkl = nullptr;            // n x 3 Miller indices (reference G order)
+  const double* kinetic_ev = nullptr;  // n
+  double ev_per_hartree = 27.211386245988;
+  std::size_t nv = 0;
+  const int* v_hkl = nullptr;          // nv x 3
+  const double* v_values = nullptr;    // nv complex (re, im)
+  std::size_t atoms = 0;
+  const long* rows = nullptr;          // projectors per atom
+  const double* amplitudes = nullptr;  // per atom row-major rows x n complex, concatenated
+  const double* d = nullptr;           // per atom row-major rows x rows complex, concatenated
+  const double* q = nullptr;
+  double regularization = 1e-12;
+  std::array<int, 3> shape{};          // <= 0: reference grid rule
+};
+
+// Device-resident restatement of PlaneWaveOperator::Impl
+// (src/plane_wave_operator.cpp:29-516): index map, V(r) on the grid, kinetic
+// and H/S diagonals, stacked projectors B (np x n, column-major), Hermitised
+// per-atom D and Q.
+class DeviceOperator {
+ public:
+  DeviceOperator(Context& ctx, const OperatorDesc& desc);
+  // which: 0 = H, 1 = S, 2 = joint pair. x, hy, sy: device pointers, ld = n.
+  void apply(int which, const double* x, long columns, double* hy, double* sy, Precision prec);
+  std::size_t n() const { return n_; }
+  std::size_t np() const { return np_; }
+  long count() const { return count_; }
+  std::array<int, 3> shape() const { return shape_; }
+  double regularization() const { return reg_; }
+  const std::vector<std::uint64_t>& indices() const { return indices_; }
+  const std::vector<double>& kinetic() const { return kinetic_; }
+  const std::vector<double>& h_diagonal() const { return h_diag_; }
+  const std::vector<double>& s_diagonal() const { return s_diag_; }
+  const double* kinetic_device() const { return kinetic_d_.p; }
+  std::vector<double> local_real_natural();
+  int nsticks() const { return grid_.dev.nsticks; }
+  Context& context() { return ctx_; }
+  std::size_t max_chunk() const { return chunk_; }
+  const GridPlanDev& grid_plan() const { return grid_.dev; }
+  // stick buffer 0 or 1 sized for `columns` columns at precision `prec`
+  void* stick_buffer(int which, std::size_t columns, Precision prec);
+
+ private:
+  void apply_chunk(int which, const double* x, long columns, double* hy, double* sy, Precision prec);
+  Context& ctx_;
+  std::size_t n_ = 0, np_ = 0, chunk_ = 256;
+  long count_ = 0;
+  std::array<int, 3> shape_{};
+  double reg_ = 0;
+  std::vector<std::uint64_t> indices_;
+  std::vector<double> kinetic_, h_diag_, s_diag_;
+  DeviceGrid grid_;
+  DevBuf<double> v64_, kinetic_d_, b64_, dblk_, qblk_;
+  DevBuf<float> v32_, b32_;
+  DevBuf<int> atom_offset_, atom_rows_;
+  DevBuf<long> block_offset_;
+  int atoms_ = 0;
+  DevBuf<char> t1_, t2_;
+  DevBuf<double> p_, w_;
+};
+
+// reference index map (src/plane_wave_operator.cpp:94-97)
+std::uint64_t grid_index(const std::array<int, 3>& shape, int h, int k, int l);
+int fast_fft_length(int minimum);
+
+}  // namespace sdft_hip

@@ -1,0 +1,311 @@
+"""ctypes binding of libsdft_hip.so (the C ABI in include/sdft_hip.h).
+
+The product path: every call goes to the HIP library. There is no CPU
+fallback; a missing or unbuilt library raises at import of this module's
+first call.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsdft_hip.so")
+_LIB = None
+
+_dp = ct.POINTER(ct.c_double)
+_ip = ct.POINTER(ct.c_int)
+_lp = ct.POINTER(ct.c_long)
+_u64p = ct.POINTER(ct.c_uint64)
+
+
+class OperatorDesc(ct.Structure):
+    _fields_ = [("n", ct.c_size_t), ("hkl", _ip), ("kinetic_ev", _dp), ("ev_per_hartree", ct.c_double),
+                ("nv", ct.c_size_t), ("v_hkl", _ip), ("v_values", _dp), ("atoms", ct.c_size_t), ("rows", _lp),
+                ("amplitudes", _dp), ("d", _dp), ("q", _dp), ("regularization", ct.c_double),
+                ("shape", ct.c_int * 3)]
+
+
+class DavidsonParams(ct.Structure):
+    _fields_ = [("roots", ct.c_long), ("max_iterations", ct.c_long), ("tolerance", ct.c_double),
+                ("subspace_factor", ct.c_long), ("search_precision", ct.c_int), ("search_handoff", ct.c_double),
+                ("stagnation_window", ct.c_int), ("verbose", ct.c_int), ("timing", ct.c_int)]
+
+
+class DavidsonResult(ct.Structure):
+    _fields_ = [("eigenvalues", _dp), ("eigenvectors", _dp), ("residual_norms", _dp), ("iterations", ct.c_long),
+                ("converged", ct.c_int), ("h_columns_search", ct.c_long), ("h_columns_exact", ct.c_long),
+                ("handoff_iteration", ct.c_long), ("seconds_total", ct.c_double), ("seconds_apply", ct.c_double),
+                ("seconds_gemm", ct.c_double), ("seconds_ritz", ct.c_double), ("seconds_ortho", ct.c_double)]
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def build():
+    subprocess.run(["make", "-s", "-j8", "-C", PKG_ROOT], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise HipError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        _LIB = ct.CDLL(LIB_PATH)
+        _LIB.sdft_hip_last_error.restype = ct.c_char_p
+    return _LIB
+
+
+def _p(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _check(rc):
+    if rc:
+        msg = lib().sdft_hip_last_error().decode()
+        if rc == 2:
+            raise ValueError(msg)
+        if rc == 3:
+            raise OverflowError(msg)
+        raise HipError(msg)
+
+
+def capabilities() -> str:
+    buf = ct.create_string_buffer(1024)
+    _check(lib().sdft_hip_capabilities(buf, ct.c_size_t(1024)))
+    return buf.value.decode()
+
+
+def enumerate_g_vectors(lattice, cutoff_ev, kfrac, ev_per_ha=27.211386245988):
+    L = lib()
+    lat = np.ascontiguousarray(lattice, dtype=np.float64)
+    kf = np.ascontiguousarray(kfrac, dtype=np.float64)
+    count = ct.c_long()
+    _check(L.sdft_hip_enumerate_g_vectors(_p(lat), ct.c_double(cutoff_ev), _p(kf), ct.c_double(ev_per_ha),
+                                          ct.c_long(0), None, None, None, ct.byref(count)))
+    n = count.value
+    hkl = np.zeros((n, 3), dtype=np.int32)
+    kin = np.zeros(n)
+    cart = np.zeros((n, 3))
+    _check(L.sdft_hip_enumerate_g_vectors(_p(lat), ct.c_double(cutoff_ev), _p(kf), ct.c_double(ev_per_ha),
+                                          ct.c_long(n), _p(hkl, _ip), _p(kin), _p(cart), ct.byref(count)))
+    return hkl, kin, cart
+
+
+def grid_indices(shape, hkl):
+    shp = np.ascontiguousarray(shape, dtype=np.int32)
+    h = np.ascontiguousarray(hkl, dtype=np.int32)
+    out = np.zeros(len(h), dtype=np.uint64)
+    _check(lib().sdft_hip_grid_indices(_p(shp, _ip), ct.c_size_t(len(h)), _p(h, _ip), _p(out, _u64p)))
+    return out
+
+
+class Context:
+    """One HIP stream + rocBLAS/rocSOLVER handle, owned by one host thread."""
+
+    def __init__(self, device: int = 0):
+        self.handle = ct.c_void_p()
+        _check(lib().sdft_hip_context_create(ct.c_int(device), ct.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            lib().sdft_hip_context_destroy(self.handle)
+            self.handle = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_name(self) -> str:
+        buf = ct.create_string_buffer(256)
+        _check(lib().sdft_hip_device_name(self.handle, buf, ct.c_size_t(256)))
+        return buf.value.decode()
+
+    def synchronize(self):
+        _check(lib().sdft_hip_synchronize(self.handle))
+
+    # ---- Accelerator mirror (host pointers)
+    def zgemm(self, aop, bop, a, b, c=None, add=False, left_identity=0, precision=0, shape=None):
+        a = np.asfortranarray(a, dtype=np.complex128)
+        b = np.asfortranarray(b, dtype=np.complex128)
+        m = a.shape[1] if aop else a.shape[0]
+        k = a.shape[0] if aop else a.shape[1]
+        n = b.shape[0] if bop else b.shape[1]
+        out = np.zeros((m, n), dtype=np.complex128, order="F") if c is None else c
+        _check(lib().sdft_hip_zgemm(self.handle, ct.c_int(aop), ct.c_int(bop), ct.c_size_t(m), ct.c_size_t(n),
+                                    ct.c_size_t(k), _p(a), ct.c_size_t(a.shape[0]), _p(b), ct.c_size_t(b.shape[0]),
+                                    _p(out), ct.c_size_t(out.shape[0]), ct.c_int(int(add)),
+                                    ct.c_uint64(left_identity), ct.c_int(precision)))
+        return out
+
+    def zgemm_strided(self, aop, bop, m, n, k, a, lda, b, ldb, c, ldc, add=False, left_identity=0, precision=0):
+        """Raw call on flat complex128 buffers (tests of padded leading dimensions)."""
+        _check(lib().sdft_hip_zgemm(self.handle, ct.c_int(aop), ct.c_int(bop), ct.c_size_t(m), ct.c_size_t(n),
+                                    ct.c_size_t(k), _p(a), ct.c_size_t(lda), _p(b), ct.c_size_t(ldb), _p(c),
+                                    ct.c_size_t(ldc), ct.c_int(int(add)), ct.c_uint64(left_identity),
+                                    ct.c_int(precision)))
+
+    def zgemm_triple_right(self, a_list, b, precision=0):
+        a_list = [np.asfortranarray(a, dtype=np.complex128) for a in a_list]
+        b = np.asfortranarray(b, dtype=np.complex128)
+        m, k = a_list[0].shape
+        n = b.shape[1]
+        outs = [np.zeros((m, n), dtype=np.complex128, order="F") for _ in range(3)]
+        aa = (_dp * 3)(*[_p(a) for a in a_list])
+        cc = (_dp * 3)(*[_p(o) for o in outs])
+        _check(lib().sdft_hip_zgemm_triple_right(self.handle, ct.c_size_t(m), ct.c_size_t(n), ct.c_size_t(k), aa,
+                                                 ct.c_size_t(m), _p(b), ct.c_size_t(k), cc, ct.c_size_t(m),
+                                                 ct.c_int(precision)))
+        return outs
+
+    def local_hamiltonian(self, shape, indices, kinetic, potential, x, operator_identity=0, precision=0):
+        shp = np.ascontiguousarray(shape, dtype=np.int32)
+        idx = np.ascontiguousarray(indices, dtype=np.uint64)
+        kin = np.ascontiguousarray(kinetic, dtype=np.float64)
+        pot = np.ascontiguousarray(potential, dtype=np.float64)
+        x = np.asfortranarray(x, dtype=np.complex128)
+        out = np.zeros_like(x, order="F")
+        _check(lib().sdft_hip_local_hamiltonian(self.handle, _p(shp, _ip), _p(idx, _u64p), ct.c_size_t(len(idx)),
+                                                _p(kin), _p(pot), _p(x), ct.c_size_t(x.shape[1]), _p(out),
+                                                ct.c_uint64(operator_identity), ct.c_int(precision)))
+        return out
+
+    def paw_pair(self, identity, b, d, q, x, projector_identity=0, precision=0):
+        b = np.asfortranarray(b, dtype=np.complex128)
+        d = np.asfortranarray(d, dtype=np.complex128)
+        q = np.asfortranarray(q, dtype=np.complex128)
+        x = np.asfortranarray(x, dtype=np.complex128)
+        npj, npw = b.shape
+        h = np.zeros((npw, x.shape[1]), dtype=np.complex128, order="F")
+        s = np.zeros_like(h)
+        handled = ct.c_int()
+        _check(lib().sdft_hip_paw_pair(self.handle, ct.c_uint64(identity), ct.c_size_t(npw), ct.c_size_t(npj),
+                                       ct.c_size_t(x.shape[1]), _p(b), _p(d), _p(q), _p(x), _p(h), _p(s),
+                                       ct.c_uint64(projector_identity), ct.c_int(precision), ct.byref(handled)))
+        return h, s
+
+    def onsite_blocks(self, rows, d_blocks, q_blocks, p, precision=0):
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        d = np.ascontiguousarray(np.concatenate([np.asfortranarray(b).ravel(order="F") for b in d_blocks]))
+        q = np.ascontiguousarray(np.concatenate([np.asfortranarray(b).ravel(order="F") for b in q_blocks]))
+        p = np.asfortranarray(p, dtype=np.complex128)
+        npj, cols = p.shape
+        w = np.zeros((npj, 2 * cols), dtype=np.complex128, order="F")
+        _check(lib().sdft_hip_onsite_blocks(self.handle, ct.c_size_t(len(rows)), _p(rows, _lp), _p(d), _p(q), _p(p),
+                                            ct.c_size_t(cols), _p(w), ct.c_int(precision)))
+        return w
+
+
+class Operator:
+    """Device-resident plane-wave operator built from a synthetic Problem."""
+
+    def __init__(self, ctx: Context, prob, shape=None):
+        self.ctx = ctx
+        self.n = prob.n
+        self._keep = dict(
+            hkl=np.ascontiguousarray(prob.hkl, dtype=np.int32),
+            kin=np.ascontiguousarray(prob.kinetic_ev, dtype=np.float64),
+            vh=np.ascontiguousarray(prob.v_hkl, dtype=np.int32),
+            vv=np.ascontiguousarray(np.stack([prob.v_values.real, prob.v_values.imag], axis=1)),
+            rows=np.ascontiguousarray(prob.atom_rows, dtype=np.int64),
+            amp=np.ascontiguousarray(np.concatenate([a.ravel() for a in prob.amplitudes])),
+            d=np.ascontiguousarray(np.concatenate([a.ravel() for a in prob.d_blocks])),
+            q=np.ascontiguousarray(np.concatenate([a.ravel() for a in prob.q_blocks])),
+        )
+        k = self._keep
+        desc = OperatorDesc()
+        desc.n = prob.n
+        desc.hkl = _p(k["hkl"], _ip)
+        desc.kinetic_ev = _p(k["kin"])
+        desc.ev_per_hartree = 27.211386245988
+        desc.nv = len(k["vh"])
+        desc.v_hkl = _p(k["vh"], _ip)
+        desc.v_values = _p(k["vv"])
+        desc.atoms = len(k["rows"])
+        desc.rows = _p(k["rows"], _lp)
+        desc.amplitudes = _p(k["amp"])
+        desc.d = _p(k["d"])
+        desc.q = _p(k["q"])
+        desc.regularization = prob.regularization
+        shp = shape if shape is not None else prob.shape
+        for a in range(3):
+            desc.shape[a] = int(shp[a])
+        self.handle = ct.c_void_p()
+        _check(lib().sdft_hip_operator_create(ctx.handle, ct.byref(desc), ct.byref(self.handle)))
+        self._keep = None
+        shape_out = (ct.c_int * 3)()
+        count = ct.c_long()
+        npj = ct.c_long()
+        sticks = ct.c_long()
+        _check(lib().sdft_hip_operator_info(self.handle, shape_out, ct.byref(count), ct.byref(npj), ct.byref(sticks),
+                                            None, None, None, None, None))
+        self.shape = tuple(shape_out)
+        self.count = count.value
+        self.np = npj.value
+        self.sticks = sticks.value
+
+    def close(self):
+        if self.handle:
+            lib().sdft_hip_operator_destroy(self.handle)
+            self.handle = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        idx = np.zeros(self.n, dtype=np.uint64)
+        kin = np.zeros(self.n)
+        vr = np.zeros(self.count)
+        hd = np.zeros(self.n)
+        sd = np.zeros(self.n)
+        shape_out = (ct.c_int * 3)()
+        _check(lib().sdft_hip_operator_info(self.handle, shape_out, None, None, None, _p(idx, _u64p), _p(kin), _p(vr),
+                                            _p(hd), _p(sd)))
+        return dict(indices=idx, kinetic=kin, local_real=vr, h_diag=hd, s_diag=sd)
+
+    def apply(self, x, which="pair", precision=0):
+        x = np.asfortranarray(x, dtype=np.complex128)
+        m = x.shape[1]
+        hy = np.zeros((self.n, m), dtype=np.complex128, order="F")
+        sy = np.zeros((self.n, m), dtype=np.complex128, order="F")
+        code = {"h": 0, "s": 1, "pair": 2}[which]
+        _check(lib().sdft_hip_operator_apply(self.handle, ct.c_int(code), _p(x), ct.c_size_t(m),
+                                             _p(hy) if code != 1 else None, _p(sy) if code != 0 else None,
+                                             ct.c_int(precision), ct.c_int(0)))
+        return {"h": hy, "s": sy, "pair": (hy, sy)}[which]
+
+    def benchmark(self, columns, which="pair", precision=0, repetitions=10):
+        ms = np.zeros(3)
+        code = {"h": 0, "s": 1, "pair": 2}[which]
+        _check(lib().sdft_hip_operator_benchmark(self.handle, ct.c_int(code), ct.c_size_t(columns),
+                                                 ct.c_int(precision), ct.c_int(repetitions), _p(ms)))
+        return dict(local_ms=ms[0], paw_ms=ms[1], apply_ms=ms[2])
+
+    def davidson(self, roots, max_iterations=256, tolerance=1e-12, initial=None, subspace_factor=8,
+                 search_precision=0, search_handoff=1e-8, stagnation_window=3, verbose=0, timing=0,
+                 want_vectors=True):
+        p = DavidsonParams(roots, max_iterations, tolerance, subspace_factor, search_precision, search_handoff,
+                           stagnation_window, verbose, timing)
+        ev = np.zeros(roots)
+        res = np.zeros(roots)
+        vec = np.zeros((self.n, roots), dtype=np.complex128, order="F") if want_vectors else None
+        r = DavidsonResult()
+        r.eigenvalues = _p(ev)
+        r.eigenvectors = _p(vec) if vec is not None else None
+        r.residual_norms = _p(res)
+        init = None if initial is None else np.asfortranarray(initial[:, :roots], dtype=np.complex128)
+        _check(lib().sdft_hip_davidson(self.handle, ct.byref(p), _p(init), ct.byref(r)))
+        return dict(eigenvalues=ev, eigenvectors=vec, residual_norms=res, iterations=r.iterations,
+                    converged=bool(r.converged), h_columns_search=r.h_columns_search,
+                    h_columns_exact=r.h_columns_exact, handoff_iteration=r.handoff_iteration,
+                    seconds_total=r.seconds_total, seconds_apply=r.seconds_apply, seconds_gemm=r.seconds_gemm,
+                    seconds_ritz=r.seconds_ritz, seconds_ortho=r.seconds_ortho)

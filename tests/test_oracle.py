@@ -1,0 +1,267 @@
+"""Pins of the CPU oracle (oracle/) against closed forms, brute force and the
+reference's own known-answer tests. CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle
+from sdft_amd import synthetic as syn
+
+EV = syn.HARTREE_EV
+
+
+# ---------------------------------------------------------------- G sphere
+def brute_force_count(lattice, ecut_ha, kfrac):
+    """Count of 0.5|G+k|^2 <= Ecut over a generous box (closed form of the
+    cutoff inequality, src/pw_core.cpp:162)."""
+    b = syn.reciprocal(np.asarray(lattice, dtype=float))
+    gmax = math.sqrt(2 * (ecut_ha + 1e-8 / EV))
+    m = [int(gmax / np.linalg.norm(b[i]) * 2) + 3 for i in range(3)]
+    hh, kk, ll = np.meshgrid(*[np.arange(-x, x + 1) for x in m], indexing="ij")
+    hkl = np.stack([hh.ravel(), kk.ravel(), ll.ravel()], axis=1)
+    q = (hkl + np.asarray(kfrac)) @ b
+    return int(np.sum(0.5 * np.einsum("ij,ij->i", q, q) <= ecut_ha + 1e-8 / EV + 1e-12))
+
+
+@pytest.mark.parametrize("config,expected", [(0, 1647), (1, 23847)])
+def test_sphere_counts_gamma(config, expected):
+    cell = syn.CONFIGS[config]["cell"]()
+    hkl, kin, _ = pyoracle.enumerate_g(cell.lattice, syn.CONFIGS[config]["ecut"] * EV + 1e-8, [0, 0, 0])
+    assert len(hkl) == expected
+    assert brute_force_count(cell.lattice, syn.CONFIGS[config]["ecut"], [0, 0, 0]) == expected
+
+
+def test_sphere_counts_slab_and_mp():
+    cell = syn.slab_cell(3)
+    counts = [len(pyoracle.enumerate_g(cell.lattice, 20 * EV + 1e-8, k)[0]) for k in ([0, 0, 0], [.5, .5, 0])]
+    assert counts == [28817, 28796]
+    si8 = syn.diamond_cell(10.26)
+    mp = [len(pyoracle.enumerate_g(si8.lattice, 10 * EV + 1e-8, k)[0]) for k in syn.mp_kpoints(4)]
+    assert min(mp) == 1622 and max(mp) == 1653 and len(set(mp)) == 3
+    for k in syn.mp_kpoints(4)[:5]:
+        assert len(pyoracle.enumerate_g(si8.lattice, 10 * EV + 1e-8, k)[0]) == brute_force_count(si8.lattice, 10.0, k)
+
+
+def test_sphere_order_is_sorted_and_unique():
+    cell = syn.diamond_cell(10.26)
+    hkl, kin, _ = pyoracle.enumerate_g(cell.lattice, 10 * EV + 1e-8, [.125, .375, -.125])
+    keys = [(k, h[0], h[1], h[2]) for k, h in zip(kin, hkl.tolist())]
+    assert keys == sorted(keys)
+    assert len(set(map(tuple, hkl.tolist()))) == len(hkl)
+
+
+def test_fast_fft_length():
+    # src/fft_planning.hpp:16-23: next 2,3,5,7-smooth length
+    assert [pyoracle.fast_fft_length(m) for m in (1, 11, 13, 29, 31, 61, 69, 71, 121, 124)] == \
+        [1, 12, 14, 30, 32, 63, 70, 72, 125, 125]
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("shape", [(8, 15, 14), (27, 25, 16), (4, 6, 7), (11, 13, 5), (30, 30, 30)])
+def test_fft3d_matches_numpy(shape):
+    rng = np.random.default_rng(7)
+    g = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    fwd = pyoracle.fft3d(g, -1)
+    bwd = pyoracle.fft3d(g, +1)
+    assert np.abs(fwd - np.fft.fftn(g)).max() < 1e-12 * np.abs(g).sum() ** 0 * 1e3
+    assert np.abs(bwd - np.fft.ifftn(g) * g.size).max() < 1e-9
+    assert np.allclose(fwd, np.fft.fftn(g), rtol=0, atol=1e-10)
+
+
+def stencil_case(shape, columns=5, npw=73):
+    """Known answer of tests/native/fft_backend_smoke.cpp:17-40: with
+    V = .4 + .13 cos(2 pi (x/N0 + 2y/N1)) + .07 sin(4 pi z/N2) the local
+    operator is a closed-form stencil."""
+    n0, n1, n2 = shape
+    count = n0 * n1 * n2
+    npw = min(npw, count)
+    idx = np.arange(npw, dtype=np.uint64)
+    kinetic = .013 * (1 + np.arange(npw) % 19)
+    x, y, z = np.meshgrid(np.arange(n0), np.arange(n1), np.arange(n2), indexing="ij")
+    pot = (.4 + .13 * np.cos(6.2831853071795864769 * (x / n0 + 2. * y / n1)) +
+           .07 * np.sin(12.5663706143591729538 * z / n2)).ravel()
+    b, g = np.meshgrid(np.arange(columns), np.arange(npw), indexing="ij")
+    inp = (np.sin(.17 * (g + 1) * (b + 2)) + 1j * np.cos(.13 * (g + 2) * (b + 1))).T.copy()  # npw x columns
+    lookup = -np.ones(count, dtype=np.int64)
+    lookup[:npw] = np.arange(npw)
+
+    def index(a, c, d):
+        return ((a % n0) * n1 + (c % n1)) * n2 + (d % n2)
+
+    expected = np.zeros_like(inp)
+    for gi in range(npw):
+        zz = gi % n2
+        yy = (gi // n2) % n1
+        xx = gi // n2 // n1
+
+        def at(a, c, d):
+            j = lookup[index(a, c, d)]
+            return inp[j] if j >= 0 else 0
+
+        expected[gi] = ((kinetic[gi] + .4) * inp[gi] + .065 * (at(xx - 1, yy - 2, zz) + at(xx + 1, yy + 2, zz)) +
+                        (-.035j) * at(xx, yy, zz - 2) + (.035j) * at(xx, yy, zz + 2))
+    return idx, kinetic, pot, inp, expected
+
+
+def oracle_local_apply(shape, idx, kinetic, pot, inp):
+    """The reference CPU loop (src/plane_wave_operator.cpp:249-255) through the oracle FFT."""
+    count = int(np.prod(shape))
+    out = np.zeros_like(inp)
+    for c in range(inp.shape[1]):
+        grid = np.zeros(count, dtype=np.complex128)
+        grid[idx.astype(np.int64)] = inp[:, c]
+        grid = pyoracle.fft3d(grid.reshape(shape), +1).ravel()
+        grid = grid * pot
+        grid = pyoracle.fft3d(grid.reshape(shape), -1).ravel()
+        out[:, c] = kinetic * inp[:, c] + grid[idx.astype(np.int64)] / count
+    return out
+
+
+@pytest.mark.parametrize("shape", [(8, 15, 14), (27, 25, 16), (4, 6, 7)])
+def test_oracle_fft_local_operator_stencil(shape):
+    idx, kin, pot, inp, expected = stencil_case(shape)
+    got = oracle_local_apply(shape, idx, kin, pot, inp)
+    scale = max(1.0, np.abs(expected).max())
+    assert np.abs(got - expected).max() / scale < 2e-11  # tests/native/fft_backend_smoke.cpp:74
+
+
+# ---------------------------------------------------------------- operator
+def smoke_problem():
+    """The dense-oracle geometry of tests/native/plane_wave_operator_smoke.cpp:16-37."""
+    hkl, kin = [], []
+    for h in range(-2, 3):
+        for k in range(-1, 2):
+            for l in range(-1, 2):
+                hkl.append((h, k, l))
+                kin.append(EV * (.8 + h * h * .6 + k * k * .9 + l * l * .7))
+    v = {(0, 0, 0): .4, (1, -1, 0): .21 + .13j, (-1, 1, 0): .21 - .13j, (4, 2, -2): .09 - .06j,
+         (-4, -2, 2): .09 + .06j, (7, 0, 0): .7 + .2j, (-7, 0, 0): .7 - .2j}
+    n, npj = len(hkl), 3
+    i, j = np.meshgrid(np.arange(npj), np.arange(n), indexing="ij")
+    b = (.04 * np.sin(.3 * (i + 1) * (j + 2)) + 1j * .03 * np.cos(.7 * (i + 3) * (j + 1)))
+    d = np.array([[.3, .04 + .06j, -.03], [.04 - .06j, -.2, .02 + .05j], [-.03, .02 - .05j, .5]])
+    q = np.array([[.06, .01 - .01j, 0], [.01 + .01j, .04, 0], [0, 0, .08]])
+    b2 = .7 * b[:2].copy()
+    b2[1] *= (.6 + .8j)
+    prob = syn.Problem(None, 0.0, np.zeros(3), (0, 0, 0), np.array(hkl, dtype=np.int32), np.array(kin),
+                       np.zeros((n, 3)), np.array(list(v.keys()), dtype=np.int32),
+                       np.array(list(v.values()), dtype=np.complex128), np.array([3, 2]),
+                       [b.astype(np.complex128), b2], [d.astype(np.complex128), d[:2, :2].astype(np.complex128)],
+                       [q.astype(np.complex128), q[:2, :2].astype(np.complex128)])
+    # dense H and S exactly as the smoke assembles them (the (7,0,0) terms
+    # lie outside every basis difference and must not alias in)
+    hd = np.zeros((n, n), dtype=np.complex128)
+    for a in range(n):
+        for c in range(n):
+            key = tuple(np.array(hkl[a]) - np.array(hkl[c]))
+            hd[a, c] = v.get(key, 0)
+    hd += np.diag(np.array(kin) / EV)
+    hd += b.conj().T @ d @ b + b2.conj().T @ d[:2, :2] @ b2
+    sd = (1 + 1e-12) * np.eye(n) + b.conj().T @ q @ b + b2.conj().T @ q[:2, :2] @ b2
+    return prob, hd, sd
+
+
+def test_smoke_geometry_grid_and_dense_operator():
+    prob, hd, sd = smoke_problem()
+    op = pyoracle.Operator(prob)
+    assert op.count == 225  # "grid=225" of the reference's recorded smoke run
+    eye = np.eye(prob.n, dtype=np.complex128)
+    hy, sy = op.apply(eye, "pair")
+    assert np.abs(hy - hd).max() <= 1e-12
+    assert np.abs(sy - sd).max() <= 1e-12
+    # Hermiticity violations are rejected (src/plane_wave_operator.cpp:439-443)
+    bad = syn.Problem(**{**prob.__dict__, "v_values": prob.v_values * np.array([1, 1 + .5j, 1, 1, 1, 1, 1])})
+    with pytest.raises(ValueError):
+        pyoracle.Operator(bad)
+
+
+def test_si8_operator_matches_dense_assembly(si8, si8_oracle):
+    h, s = syn.dense_operator(si8)
+    x = syn.initial_psi(si8, 7, 0)
+    hy, sy = si8_oracle.apply(x, "pair")
+    assert np.abs(hy - h @ x).max() <= 1e-12 * max(1, np.abs(h @ x).max())
+    assert np.abs(sy - s @ x).max() <= 1e-12
+    # the same operator on the reference grid rule (30^3): grid independence
+    ref = pyoracle.Operator(si8, shape=(0, 0, 0))
+    assert ref.shape == (30, 30, 30)
+    assert np.abs(ref.apply(x, "h") - hy).max() < 1e-12
+
+
+def test_overlap_spectrum_window(si8):
+    h, s = syn.dense_operator(si8)
+    w = np.linalg.eigvalsh(s)
+    assert w.min() >= 0.8 - 1e-9 and w.max() <= 1.5 + 1e-9
+
+
+def test_compensated_product_is_exactly_rounded():
+    """src/paw_compensated_product.hpp: TwoSum of FMA-exact products; compare
+    with exact rational arithmetic (the result is the correctly rounded sum
+    up to the final sum+correction rounding)."""
+    from fractions import Fraction
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6))
+    b = rng.standard_normal((6, 4)) + 1j * rng.standard_normal((6, 4))
+    # cancellation-heavy: make column 0 nearly annihilated
+    b[:, 0] = np.linalg.solve(a, np.full(6, 1e-9 + 0j))
+    c = pyoracle.compensated_product(0, a, b)
+    for i in range(6):
+        for j in range(4):
+            re = sum(Fraction(a[i, k].real) * Fraction(b[k, j].real) - Fraction(a[i, k].imag) * Fraction(b[k, j].imag)
+                     for k in range(6))
+            im = sum(Fraction(a[i, k].real) * Fraction(b[k, j].imag) + Fraction(a[i, k].imag) * Fraction(b[k, j].real)
+                     for k in range(6))
+            assert abs(c[i, j].real - float(re)) <= 2 * np.spacing(abs(float(re)) + 1e-300)
+            assert abs(c[i, j].imag - float(im)) <= 2 * np.spacing(abs(float(im)) + 1e-300)
+
+
+@pytest.mark.parametrize("e", [0., .01, 1., 100.])
+@pytest.mark.parametrize("t", [0., .001, 1., 100., 1e100])
+def test_kinetic_preconditioner_closed_form(e, t):
+    # tests/native/davidson_optimization_smoke.cpp:12-24
+    scale = 1.5 * max(e, 2. / 27.211386245988)
+    x = t / scale
+    f = pyoracle.kinetic_preconditioner(t, e)
+    assert math.isfinite(f) and 0 <= f <= 2. / scale
+    if x < 1e20:
+        p = 27. + 18. * x + 12. * x * x + 8. * x * x * x
+        expected = (2. / scale) * p / (p + 16. * x ** 4)
+        assert abs(f - expected) <= 1e-13 * expected
+    with pytest.raises(ValueError):
+        pyoracle.kinetic_preconditioner(-1., 1.)
+
+
+def test_generalized_eigensolver_against_scipy():
+    import scipy.linalg as sl
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 7, 40):
+        a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        a = a + a.conj().T
+        m = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        b = m @ m.conj().T + n * np.eye(n)
+        w, v = pyoracle.generalized_eigen(a, b)
+        ref = sl.eigh(a, b, eigvals_only=True)
+        assert np.abs(w - ref).max() < 1e-11 * max(1, np.abs(ref).max())
+        assert np.abs(v.conj().T @ b @ v - np.eye(n)).max() < 1e-11
+        assert np.abs(a @ v - b @ v * w).max() < 1e-10 * max(1, np.abs(ref).max())
+
+
+def test_oracle_davidson_si8_against_dense(si8, si8_oracle):
+    """Davidson restatement vs a dense FP64 generalized eigensolve; fresh
+    residuals re-verified against the dense operator (docs/THEORY.md §9)."""
+    import scipy.linalg as sl
+    h, s = syn.dense_operator(si8)
+    ref = sl.eigh(h, s, eigvals_only=True, subset_by_index=[0, 15])
+    init = syn.initial_psi(si8, 32, 0)
+    r = si8_oracle.davidson(16, 256, 1e-12, init)
+    assert r["converged"]
+    assert np.abs(r["eigenvalues"] - ref).max() < 1e-10
+    x = r["eigenvectors"]
+    res = np.linalg.norm(h @ x - (s @ x) * r["eigenvalues"], axis=0)
+    assert res.max() <= 1.01e-12
+    assert np.abs(x.conj().T @ s @ x - np.eye(16)).max() < 1e-10
+
+
+def test_oracle_davidson_iteration_limited_is_not_converged(si8_oracle):
+    r = si8_oracle.davidson(8, 1, 1e-12)
+    assert not r["converged"]
