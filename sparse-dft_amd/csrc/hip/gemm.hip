@@ -122,7 +122,9 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, const TA* __
     }
     __syncthreads();
   }
-  // D layout: lane holds rows 4*(lane/16) + r, column lane % 16 of each 16x16 block
+  // Accumulator layout of each 16x16 block, column lane % 16 in both cases:
+  //   f32 16x16x4: register r holds row 4*(lane/16) + r
+  //   f64 16x16x4: register r holds row 4*r + lane/16
   const bool split = work != nullptr;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -130,7 +132,8 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, const TA* __
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        const int row = sizeof(T) == 8 ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r;
+        const int m = m0 + wm + 16 * i + row;
         const int n = n0 + wn + 16 * j + (lane & 15);
         if (m >= M || n >= N) continue;
         double re = alpha * double(acc_re[i][j][r]), im = alpha * double(acc_im[i][j][r]);

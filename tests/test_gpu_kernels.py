@@ -138,8 +138,9 @@ def test_onsite_blocks_bitwise_against_oracle(hip_ctx, si8):
     for r, db, qb in zip(si8.atom_rows, d, q):
         ed = pyoracle.compensated_product(0, db, p[o:o + r])
         eq = pyoracle.compensated_product(0, qb, p[o:o + r])
-        assert np.array_equal(w[o:o + r, :9].view(np.uint64), ed.view(np.uint64))
-        assert np.array_equal(w[o:o + r, 9:].view(np.uint64), eq.view(np.uint64))
+        bits = lambda a: np.ascontiguousarray(a).view(np.uint64)  # noqa: E731
+        assert np.array_equal(bits(w[o:o + r, :9]), bits(ed))
+        assert np.array_equal(bits(w[o:o + r, 9:]), bits(eq))
         o += r
 
 
