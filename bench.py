@@ -66,6 +66,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--timing", type=int, default=0, help="synchronise and time Davidson phases")
+    ap.add_argument("--subspace-factor", type=int, default=6,
+                    help="Davidson capacity factor (the reference's SDFT_DAVIDSON_SUBSPACE_FACTOR, 4..16)")
+    ap.add_argument("--stagnation-window", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by k-point/request "
@@ -86,7 +90,9 @@ def main():
     def solve(precision):
         ctx.synchronize()
         t0 = time.perf_counter()
-        r = op.davidson(ROOTS, 256, 1e-12, init, search_precision=precision, verbose=args.verbose)
+        r = op.davidson(ROOTS, 256, 1e-12, init, search_precision=precision, verbose=args.verbose,
+                        timing=args.timing, subspace_factor=args.subspace_factor,
+                        stagnation_window=args.stagnation_window, handoff_restart=1)
         ctx.synchronize()
         r["wall"] = time.perf_counter() - t0
         if not r["converged"]:
@@ -130,7 +136,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "Si64 Gamma block Davidson, 160 bands, FP64-certified <= 1e-12 Ha",
                    "npw": npw, "grid": list(prob.shape), "projectors": prob.np_total, "roots": ROOTS,
-                   "ecut_ha": 15.0, "parallelism": "single GPU"},
+                   "ecut_ha": 15.0, "subspace_factor": args.subspace_factor, "parallelism": "single GPU"},
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "fp64_path": {"value": ROOTS / fp64_wall, "latency_s": fp64_wall, "iterations": fp64[-1]["iterations"]},
@@ -146,6 +152,10 @@ def main():
                      "algorithmic_bytes": bytes_alg, "ms_per_launch": bench["local_ms"],
                      "fp64": {"achieved": achieved64, "ms_per_launch": bench64["local_ms"]}},
     }
+    if args.timing:
+        keys = ("seconds_total", "seconds_apply", "seconds_gemm", "seconds_ritz", "seconds_ortho",
+                "h_columns_search", "h_columns_exact")
+        out["phases"] = {"fp32_search": {k: runs[-1][k] for k in keys}, "fp64": {k: fp64[-1][k] for k in keys}}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prob, fp64[-1]["iterations"])
     print(json.dumps(out))

@@ -137,6 +137,7 @@ typedef struct {
   int search_precision;
   double search_handoff;
   int stagnation_window;
+  int handoff_restart;
   int verbose;
   int timing;
 } sdft_hip_davidson_params;

@@ -472,6 +472,7 @@ int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* par
     p.search_fp32 = params->search_precision;
     p.search_handoff = params->search_handoff > 0 ? params->search_handoff : 1e-8;
     p.stagnation_window = params->stagnation_window > 0 ? params->stagnation_window : 3;
+    p.handoff_restart = params->handoff_restart;
     p.verbose = params->verbose;
     p.timing = params->timing;
     if (p.search_fp32 != 0 && p.search_fp32 != 1) throw std::invalid_argument("HIP: search precision must be 0 or 1");

@@ -122,16 +122,39 @@ class Solver {
     if (*reinterpret_cast<int*>(h.data())) throw std::runtime_error("HIP: Hermitian eigensolve failed");
     return std::vector<double>(h.begin() + 8, h.begin() + 8 + m);
   }
-  // generalized A v = l B v (itype 1), A overwritten by B-orthonormal vectors
+  // Generalized A v = l B v (itype 1), A overwritten by B-orthonormal vectors:
+  // B = L L^H (potrf), C = L^{-1} A L^{-H} (two BLAS3 triangular solves),
+  // C = Y diag(l) Y^H (divide and conquer), v = L^{-H} Y. The same reduction as
+  // LAPACK zhegv/zhegvd, with the two-sided solve done by ztrsm.
   std::vector<double> hegvd(double* a, double* b, long m, long ld) {
     double* w = small_.p + 8;
     double* e = w + m;
     int* info = reinterpret_cast<int*>(small_.p);
     const double t0 = p_.timing ? (sync(), now()) : 0;
-    check(rocsolver_zhegvd(ctx_.blas(), rocblas_eform_ax, rocblas_evect_original, rocblas_fill_lower,
-                           rocblas_int(m), reinterpret_cast<rocblas_double_complex*>(a), rocblas_int(ld),
-                           reinterpret_cast<rocblas_double_complex*>(b), rocblas_int(ld), w, e, info),
-          "rocsolver_zhegvd");
+    auto* A = reinterpret_cast<rocblas_double_complex*>(a);
+    auto* B = reinterpret_cast<rocblas_double_complex*>(b);
+    const rocblas_double_complex one{1.0, 0.0};
+    check(rocsolver_zpotrf(ctx_.blas(), rocblas_fill_lower, rocblas_int(m), B, rocblas_int(ld), info),
+          "rocsolver_zpotrf");
+    int pinfo = 0;
+    check(hipMemcpyAsync(&pinfo, info, sizeof(int), hipMemcpyDeviceToHost, s_), "info copy");
+    sync();
+    if (pinfo) throw std::runtime_error("HIP: Davidson Rayleigh-Ritz metric is not positive definite");
+    check(rocblas_ztrsm(ctx_.blas(), rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                        rocblas_diagonal_non_unit, rocblas_int(m), rocblas_int(m), &one, B, rocblas_int(ld), A,
+                        rocblas_int(ld)),
+          "ztrsm");
+    check(rocblas_ztrsm(ctx_.blas(), rocblas_side_right, rocblas_fill_lower, rocblas_operation_conjugate_transpose,
+                        rocblas_diagonal_non_unit, rocblas_int(m), rocblas_int(m), &one, B, rocblas_int(ld), A,
+                        rocblas_int(ld)),
+          "ztrsm");
+    check(rocsolver_zheevd(ctx_.blas(), rocblas_evect_original, rocblas_fill_lower, rocblas_int(m), A,
+                           rocblas_int(ld), w, e, info),
+          "rocsolver_zheevd");
+    check(rocblas_ztrsm(ctx_.blas(), rocblas_side_left, rocblas_fill_lower, rocblas_operation_conjugate_transpose,
+                        rocblas_diagonal_non_unit, rocblas_int(m), rocblas_int(m), &one, B, rocblas_int(ld), A,
+                        rocblas_int(ld)),
+          "ztrsm");
     std::vector<double> h = to_host(small_.p, 8 + m);
     if (p_.timing) stats_.seconds_ritz += now() - t0;
     if (*reinterpret_cast<int*>(h.data())) throw std::runtime_error("HIP: Davidson Rayleigh-Ritz failed");
@@ -528,12 +551,26 @@ DavidsonOutput Solver::run(const C* initial) {
     if (!exact_) {
       // FP32 search hand-off: the reference's 1e-8 threshold, or a stalled
       // FP32 residual (its rounding floor lies far above 1e-8)
-      if (max_res() < 0.5 * best_history) { best_history = max_res(); since_halving = 0; } else ++since_halving;
+      // stagnation counts only near the FP32 rounding floor (below 1e-3), so
+      // the slow first iterations of a cold block never trigger it
+      if (max_res() < 0.5 * best_history) { best_history = max_res(); since_halving = 0; } else if (max_res() < 1e-3) ++since_halving;
       const bool stalled = since_halving >= p_.stagnation_window;
       if (max_res() < p_.search_handoff || it + 1 == p_.max_iterations || stalled) {
-        exact_ = true;
         stats_.handoff_iteration = it + 1;
-        rebuild_exact();
+        if (p_.handoff_restart && size_ > 2 * R) {
+          // keep the 2 roots lowest Ritz vectors (FP64 rotation), then the
+          // exact block CGS2 and the next flush give them FP64 S and H images
+          const long keep = std::min(size_, 2 * R);
+          exact_ = true;
+          gemm(true, 0, 0, n, keep, size_, 1.0, basis_.p, n, A, size_, retained.p, n);
+          size_ = 0;
+          computed_h_ = 0;
+          projected_size_ = 0;
+          append_block(retained.p, keep);
+        } else {
+          exact_ = true;
+          rebuild_exact();
+        }
         continue;
       }
     }

@@ -32,7 +32,8 @@ class OperatorDesc(ct.Structure):
 class DavidsonParams(ct.Structure):
     _fields_ = [("roots", ct.c_long), ("max_iterations", ct.c_long), ("tolerance", ct.c_double),
                 ("subspace_factor", ct.c_long), ("search_precision", ct.c_int), ("search_handoff", ct.c_double),
-                ("stagnation_window", ct.c_int), ("verbose", ct.c_int), ("timing", ct.c_int)]
+                ("stagnation_window", ct.c_int), ("handoff_restart", ct.c_int), ("verbose", ct.c_int),
+                ("timing", ct.c_int)]
 
 
 class DavidsonResult(ct.Structure):
@@ -291,10 +292,10 @@ class Operator:
         return dict(local_ms=ms[0], paw_ms=ms[1], apply_ms=ms[2])
 
     def davidson(self, roots, max_iterations=256, tolerance=1e-12, initial=None, subspace_factor=8,
-                 search_precision=0, search_handoff=1e-8, stagnation_window=3, verbose=0, timing=0,
-                 want_vectors=True):
+                 search_precision=0, search_handoff=1e-8, stagnation_window=3, handoff_restart=1, verbose=0,
+                 timing=0, want_vectors=True):
         p = DavidsonParams(roots, max_iterations, tolerance, subspace_factor, search_precision, search_handoff,
-                           stagnation_window, verbose, timing)
+                           stagnation_window, handoff_restart, verbose, timing)
         ev = np.zeros(roots)
         res = np.zeros(roots)
         vec = np.zeros((self.n, roots), dtype=np.complex128, order="F") if want_vectors else None

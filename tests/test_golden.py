@@ -1,0 +1,72 @@
+"""The oracle and the product's host code against golden vectors written from
+the reference's OWN compiled sources (tests/golden/make_golden.py over
+oracle/_ref, built by oracle/build_ref.sh). CPU only."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle
+from sdft_amd import native
+from sdft_amd import synthetic as syn
+
+GOLDEN = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_golden.npz"))
+EV = syn.HARTREE_EV
+
+
+def digest(hkl, kin):
+    return hashlib.sha256(np.ascontiguousarray(hkl, dtype=np.int32).tobytes() +
+                          np.ascontiguousarray(kin, dtype=np.float64).tobytes()).hexdigest()
+
+
+CASES = {"si8_gamma": (lambda: syn.diamond_cell(10.26).lattice, 10.0),
+         "si8_mp444": (lambda: syn.diamond_cell(10.26).lattice, 10.0),
+         "si64_gamma": (lambda: syn.diamond_cell(10.26, (2, 2, 2)).lattice, 15.0),
+         "slab_2k": (lambda: syn.slab_cell(3).lattice, 20.0)}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_sphere_order_bit_exact_with_reference(name, impl):
+    lat_fn, ecut = CASES[name]
+    lat = lat_fn()
+    fn = pyoracle.enumerate_g if impl == "oracle" else native.enumerate_g_vectors
+    for k, want, count in zip(GOLDEN[f"{name}_kpoints"], GOLDEN[f"{name}_sha256"], GOLDEN[f"{name}_counts"]):
+        hkl, kin, _ = fn(lat, ecut * EV + 1e-8, k)
+        assert len(hkl) == count
+        assert digest(hkl, kin) == str(want), f"{name} k={k}"
+
+
+def test_si8_sphere_full_list():
+    hkl, kin, _ = native.enumerate_g_vectors(syn.diamond_cell(10.26).lattice, 10 * EV + 1e-8, [0, 0, 0])
+    assert np.array_equal(hkl, GOLDEN["si8_gamma_hkl"])
+    assert np.array_equal(kin.view(np.uint64), GOLDEN["si8_gamma_kinetic_ev"].view(np.uint64))
+
+
+def test_compensated_product_bitwise_with_reference():
+    a, b = GOLDEN["comp_a"], GOLDEN["comp_b"]
+    for aop, key in ((0, "comp_normal"), (1, "comp_adjoint")):
+        got = pyoracle.compensated_product(aop, a, b)
+        assert np.array_equal(np.ascontiguousarray(got).view(np.uint64),
+                              np.ascontiguousarray(GOLDEN[key]).view(np.uint64))
+
+
+def test_kinetic_preconditioner_bitwise_with_reference():
+    for i, t in enumerate(GOLDEN["kp_t"]):
+        for j, e in enumerate(GOLDEN["kp_e"]):
+            assert pyoracle.kinetic_preconditioner(t, e) == GOLDEN["kp_values"][i, j]
+
+
+def test_oracle_davidson_matches_reference_davidson(si8_oracle, si8):
+    """Same operator, same warm start: the oracle restatement and the
+    reference's compiled Davidson reach the same eigenvalues in the same
+    number of iterations."""
+    init = syn.initial_psi(si8, 32, 0)
+    r = si8_oracle.davidson(16, 256, 1e-12, init)
+    assert r["converged"]
+    assert np.abs(r["eigenvalues"] - GOLDEN["si8_davidson16_eigenvalues"]).max() < 1e-12
+    assert r["iterations"] == int(GOLDEN["si8_davidson16_iterations"])
+    r8 = si8_oracle.davidson(8, 256, 1e-12)
+    assert np.abs(r8["eigenvalues"] - GOLDEN["si8_cold8_eigenvalues"]).max() < 1e-12
+    assert r8["iterations"] == int(GOLDEN["si8_cold8_iterations"])

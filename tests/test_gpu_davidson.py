@@ -45,6 +45,20 @@ def test_davidson_si8_certified(si8, si8_dev, si8_dense, search_precision):
         assert r["handoff_iteration"] > 0 and r["h_columns_search"] > 0
 
 
+def test_davidson_si8_matches_reference_golden(si8, si8_dev):
+    """Eigenvalues of the reference's own compiled Davidson on the same
+    operator and warm start (tests/golden/make_golden.py)."""
+    import os
+    golden = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_golden.npz"))
+    init = syn.initial_psi(si8, 32, 0)
+    for precision in (0, 1):
+        r = si8_dev.davidson(16, 256, 1e-12, init, search_precision=precision)
+        assert r["converged"]
+        assert np.abs(r["eigenvalues"] - golden["si8_davidson16_eigenvalues"]).max() < 1e-10
+    r = si8_dev.davidson(8, 256, 1e-12)
+    assert np.abs(r["eigenvalues"] - golden["si8_cold8_eigenvalues"]).max() < 1e-10
+
+
 def test_davidson_cold_start_si8(si8_dev, si8_dense, si8):
     h, s, w = si8_dense
     r = si8_dev.davidson(8, 256, 1e-12)
