@@ -124,6 +124,12 @@ struct DavidsonResult {
 DavidsonResult davidson(const Operator& op, std::size_t roots, std::size_t max_iterations,
                         double tolerance, const std::vector<C>& initial, std::size_t subspace_factor = 8);
 
+// Binds CBLAS zgemm / LAPACKE zhegvd, zheevd (ILP64, numpy's OpenBLAS naming)
+// from the shared library at `path`; an empty path restores the plain loops.
+bool bind_blas(const char* path);
+bool lapack_hegvd(std::size_t n, C* a, C* b, double* w);  // false when unbound
+bool lapack_heevd(std::size_t n, C* a, double* w);
+
 // Dense generalized Hermitian eigenproblem A v = lambda B v (B HPD), all
 // eigenpairs ascending; eigenvectors column-major B-orthonormal.
 void hermitian_generalized_eigen(std::size_t n, const C* a, const C* b, double* values, C* vectors);

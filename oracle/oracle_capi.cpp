@@ -61,6 +61,12 @@ int orc_reciprocal(const double* lattice, double* out) {
 
 int orc_fast_fft_length(int minimum) { return orc::fast_fft_length(minimum); }
 
+int orc_bind_blas(const char* path) {
+  return guarded([&] {
+    if (!orc::bind_blas(path)) throw std::runtime_error(std::string("cannot bind BLAS from ") + (path ? path : ""));
+  });
+}
+
 // In-place unnormalised 3-D FFT of a row-major grid (sign -1 forward, +1 backward).
 int orc_fft3d(const int* shape, double* grid, int sign) {
   return guarded([&] {

@@ -2,8 +2,11 @@
 // TEST INFRASTRUCTURE ONLY (see oracle.hpp). Built with -ffp-contract=off.
 #include "oracle.hpp"
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -155,11 +158,62 @@ void Fft3d::exec(C* g, int sign) const {
   }
 }
 
+// ---------------------------------------------------------------- BLAS
+// Optional CBLAS/LAPACKE (ILP64, as bundled with numpy) bound at run time. The
+// reference's CPU path calls cblas_zgemm and zhegvd when they are available
+// (src/compute_backend.cpp:182-184, src/pw_core.cpp:104-122); the timed CPU
+// baseline binds them, the parity tests keep the plain loops below.
+namespace {
+using zgemm64_t = void (*)(int, int, int, std::int64_t, std::int64_t, std::int64_t, const void*, const void*,
+                           std::int64_t, const void*, std::int64_t, const void*, void*, std::int64_t);
+using zhegvd64_t = std::int64_t (*)(int, std::int64_t, char, char, std::int64_t, void*, std::int64_t, void*,
+                                    std::int64_t, double*);
+using zheevd64_t = std::int64_t (*)(int, char, char, std::int64_t, void*, std::int64_t, double*);
+zgemm64_t g_zgemm = nullptr;
+zhegvd64_t g_zhegvd = nullptr;
+zheevd64_t g_zheevd = nullptr;
+}  // namespace
+
+bool bind_blas(const char* path) {
+  if (!path || !*path) {
+    g_zgemm = nullptr;
+    g_zhegvd = nullptr;
+    g_zheevd = nullptr;
+    return true;
+  }
+  void* h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+  if (!h) return false;
+  g_zgemm = reinterpret_cast<zgemm64_t>(dlsym(h, "scipy_cblas_zgemm64_"));
+  g_zhegvd = reinterpret_cast<zhegvd64_t>(dlsym(h, "scipy_LAPACKE_zhegvd64_"));
+  g_zheevd = reinterpret_cast<zheevd64_t>(dlsym(h, "scipy_LAPACKE_zheevd64_"));
+  return g_zgemm && g_zhegvd && g_zheevd;
+}
+
+bool lapack_hegvd(std::size_t n, C* a, C* b, double* w) {
+  if (!g_zhegvd) return false;
+  const auto info = g_zhegvd(102, 1, 'V', 'L', std::int64_t(n), a, std::int64_t(n), b, std::int64_t(n), w);
+  if (info) throw std::runtime_error("LAPACK zhegvd failed");
+  return true;
+}
+
+bool lapack_heevd(std::size_t n, C* a, double* w) {
+  if (!g_zheevd) return false;
+  const auto info = g_zheevd(102, 'V', 'L', std::int64_t(n), a, std::int64_t(n), w);
+  if (info) throw std::runtime_error("LAPACK zheevd failed");
+  return true;
+}
+
 // ---------------------------------------------------------------- GEMM
 void zgemm(Op ao, Op bo, std::size_t m, std::size_t n, std::size_t k, const C* a, std::size_t lda,
            const C* b, std::size_t ldb, C* c, std::size_t ldc, bool add) {
   if (!m || !n || !k || lda < (ao == Op::Normal ? m : k) || ldb < (bo == Op::Normal ? k : n) || ldc < m)
     throw std::invalid_argument("invalid complex GEMM shape/leading dimension");
+  if (g_zgemm) {
+    const C one(1), beta(add ? 1. : 0.);
+    g_zgemm(102, ao == Op::Normal ? 111 : 113, bo == Op::Normal ? 111 : 113, std::int64_t(m), std::int64_t(n),
+            std::int64_t(k), &one, a, std::int64_t(lda), b, std::int64_t(ldb), &beta, c, std::int64_t(ldc));
+    return;
+  }
 #pragma omp parallel for schedule(static) if (double(m) * n * k > 1e5)
   for (std::ptrdiff_t j = 0; j < std::ptrdiff_t(n); ++j) {
     std::vector<C> col(m, C(0));

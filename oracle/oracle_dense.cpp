@@ -77,6 +77,13 @@ void tridiagonal_ql(std::size_t n, std::vector<double>& d, std::vector<double>& 
 
 void hermitian_eigen(std::size_t n, const C* a_in, double* values, C* vectors) {
   if (!n) return;
+  {
+    std::vector<C> a(a_in, a_in + n * n);
+    if (lapack_heevd(n, a.data(), values)) {
+      std::copy(a.begin(), a.end(), vectors);
+      return;
+    }
+  }
   std::vector<C> a(a_in, a_in + n * n);  // column-major, lower triangle used after symmetrisation
   for (std::size_t j = 0; j < n; ++j)
     for (std::size_t i = j; i < n; ++i) {
@@ -165,6 +172,13 @@ void hermitian_eigen(std::size_t n, const C* a_in, double* values, C* vectors) {
 
 void hermitian_generalized_eigen(std::size_t n, const C* a, const C* b, double* values, C* vectors) {
   if (!n) return;
+  {
+    std::vector<C> aa(a, a + n * n), bb(b, b + n * n);
+    if (lapack_hegvd(n, aa.data(), bb.data(), values)) {
+      std::copy(aa.begin(), aa.end(), vectors);
+      return;
+    }
+  }
   // Cholesky B = L L^H (lower, column-major)
   std::vector<C> l(n * n, C(0));
   for (std::size_t j = 0; j < n; ++j) {

@@ -60,6 +60,21 @@ def enumerate_g(lattice, cutoff_ev, kfrac, ev_per_ha=27.211386245988):
     return hkl, kin, cart
 
 
+def numpy_openblas_path():
+    """numpy's bundled ILP64 OpenBLAS (CBLAS + LAPACKE), or None."""
+    import glob
+    d = os.path.join(os.path.dirname(os.path.dirname(np.__file__)), "numpy.libs")
+    hits = sorted(glob.glob(os.path.join(d, "libscipy_openblas64_*.so")))
+    return hits[0] if hits else None
+
+
+def bind_blas(path):
+    """Route the oracle's GEMMs and dense eigensolves through CBLAS/LAPACKE
+    (the reference's CPU configuration with BLAS/LAPACK); None restores the
+    plain loops."""
+    _check(lib().orc_bind_blas(None if path is None else path.encode()))
+
+
 def fast_fft_length(m):
     return lib().orc_fast_fft_length(ct.c_int(m))
 

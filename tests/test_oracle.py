@@ -265,3 +265,32 @@ def test_oracle_davidson_si8_against_dense(si8, si8_oracle):
 def test_oracle_davidson_iteration_limited_is_not_converged(si8_oracle):
     r = si8_oracle.davidson(8, 1, 1e-12)
     assert not r["converged"]
+
+
+def test_blas_bound_oracle_matches_plain_loops(si8, si8_oracle):
+    """The timed CPU baseline binds numpy's OpenBLAS (CBLAS zgemm, LAPACKE
+    zhegvd/zheevd); its GEMM, dense eigensolve and a truncated Davidson must
+    agree with the plain-loop oracle that the parity tests use."""
+    path = pyoracle.numpy_openblas_path()
+    if path is None:
+        pytest.skip("numpy ships no ILP64 OpenBLAS here")
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((37, 23)) + 1j * rng.standard_normal((37, 23))
+    b = rng.standard_normal((37, 11)) + 1j * rng.standard_normal((37, 11))
+    h = rng.standard_normal((19, 19)) + 1j * rng.standard_normal((19, 19))
+    h = h + h.conj().T
+    m = rng.standard_normal((19, 19)) + 1j * rng.standard_normal((19, 19))
+    s = m @ m.conj().T + 19 * np.eye(19)
+    init = syn.initial_psi(si8, 16, 0)
+    plain = (pyoracle.zgemm(1, 0, a, b), pyoracle.generalized_eigen(h, s)[0],
+             si8_oracle.davidson(16, 3, 1e-12, init))
+    pyoracle.bind_blas(path)
+    try:
+        bound = (pyoracle.zgemm(1, 0, a, b), pyoracle.generalized_eigen(h, s)[0],
+                 si8_oracle.davidson(16, 3, 1e-12, init))
+    finally:
+        pyoracle.bind_blas(None)
+    assert np.abs(bound[0] - a.conj().T @ b).max() < 1e-12
+    assert np.abs(bound[0] - plain[0]).max() < 1e-12
+    assert np.abs(bound[1] - plain[1]).max() < 1e-11
+    assert np.abs(bound[2]["eigenvalues"] - plain[2]["eigenvalues"]).max() < 1e-9
