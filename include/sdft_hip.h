@@ -45,6 +45,20 @@ int sdft_hip_context_destroy(sdft_hip_context* ctx);
 int sdft_hip_device_name(sdft_hip_context* ctx, char* buffer, size_t size);
 int sdft_hip_synchronize(sdft_hip_context* ctx);
 
+/* Kernel clock (measurement; no counterpart in the reference): while enabled,
+ * HIP events are recorded on the context's stream around every MFMA complex
+ * GEMM kernel and every local-Hamiltonian pipeline launch. Enabling or
+ * disabling clears the sums. kind: 0 GEMM FP64, 1 GEMM FP32 arithmetic,
+ * 2 local H FP64, 3 local H FP32. work: flops (8 m n k per GEMM) or the
+ * algorithmic bytes m (2 npw c + 4 N c) + N r + npw (4 + r) per local-H launch.
+ * The stats call synchronises the stream. */
+#define SDFT_HIP_CLOCK_GEMM_FP64 0
+#define SDFT_HIP_CLOCK_GEMM_FP32 1
+#define SDFT_HIP_CLOCK_LOCAL_FP64 2
+#define SDFT_HIP_CLOCK_LOCAL_FP32 3
+int sdft_hip_kernel_clock(sdft_hip_context* ctx, int enable);
+int sdft_hip_kernel_clock_stats(sdft_hip_context* ctx, int kind, long* launches, double* milliseconds, double* work);
+
 /* Accelerator::gemm / gemm_immutable_left (src/compute_backend.hpp:67-73):
  * C = op(A) op(B) (+ C if add), ops 0 = Normal, 1 = Adjoint, host pointers.
  * left_identity != 0 keeps op(A) resident on the device for that identity
@@ -140,6 +154,10 @@ typedef struct {
   int handoff_restart;
   int verbose;
   int timing;
+  /* > 0: the bounded band-group solver with groups of this many roots
+   * (SDFT_BAND_GROUP_SOLVER=1 with SDFT_BAND_GROUP_SIZE, src/pw_core.cpp:564-567,
+   * src/band_group_solver.cpp:57-211); 0: one block Davidson. */
+  long band_group_size;
 } sdft_hip_davidson_params;
 
 typedef struct {
@@ -152,7 +170,56 @@ typedef struct {
   long h_columns_exact;
   long handoff_iteration;
   double seconds_total, seconds_apply, seconds_gemm, seconds_ritz, seconds_ortho;
+  long cholesky_blocks; /* correction blocks normalised by Cholesky QR */
+  long rr_cholesky;     /* Rayleigh-Ritz solves that took the Cholesky reduction */
 } sdft_hip_davidson_result;
+
+/* PlaneWaveOperator::project_into / projector_metric_into
+ * (src/plane_wave_operator.cpp:375-389), host pointers: P = B X (projectors x
+ * columns, column-major) and QP = Q P with the per-atom onsite metric in the
+ * compensated FP64 order (dense Q for a PWO2 operator). */
+int sdft_hip_operator_project(sdft_hip_operator* op, const double* x, size_t columns, double* p, int precision);
+int sdft_hip_operator_projector_metric(sdft_hip_operator* op, const double* p, size_t columns, double* qp);
+
+/* plane_wave_density_fft (src/density_functionals.cpp:406-460): Fourier
+ * coefficients (nsupport complex) at the Miller triples support_hkl of
+ * rho(r) = sum_b occupations[b] / volume |psi_b(r)|^2, psi_b the band
+ * coefficients (host, n x bands complex, the operator's basis order); the
+ * transforms run on the smallest device FFT grid that is alias free. */
+int sdft_hip_operator_density(sdft_hip_operator* op, const double* coefficients, size_t bands,
+                              const double* occupations, double volume, size_t nsupport, const int* support_hkl,
+                              double* out);
+
+/* Solver snapshots SDFTPWO1/2/3 (src/plane_wave_operator.cpp:520-653): the
+ * operator arrays (indices, kinetic, H/S diagonals, V(r), B and the onsite
+ * matrices in atom-block, dense or folded form) and a Davidson request
+ * (roots, iterations, tolerance, warm block), as the replay harness of
+ * src/solver_replay_cli.cpp consumes them. */
+typedef struct {
+  int format; /* 1, 2 or 3 (SDFTPWO1/2/3) */
+  size_t n, count, shape[3], projectors, atoms;
+  size_t roots, iterations, initial_columns;
+  double regularization, tolerance;
+} sdft_hip_snapshot_info;
+/* Reads and validates the whole file as the reference loader does; no GPU. */
+int sdft_hip_snapshot_info_read(const char* path, sdft_hip_snapshot_info* info);
+/* Device operator from a snapshot; the stored warm block (n x initial_columns
+ * complex) is copied to initial when not NULL. */
+int sdft_hip_snapshot_load(sdft_hip_context* ctx, const char* path, double* initial, sdft_hip_operator** out);
+/* Writes the operator in its snapshot form (PWO3 for atom blocks) with a
+ * Davidson request; initial is n x roots complex or NULL. Refuses to
+ * overwrite an existing file. */
+int sdft_hip_snapshot_save(sdft_hip_operator* op, const char* path, size_t roots, size_t iterations,
+                           double tolerance, const double* initial);
+
+/* Dense Hermitian eigensolver of the Rayleigh-Ritz step (no counterpart in the
+ * reference, whose Ritz solve is LAPACK zhegvd; exposed for tests): all n
+ * eigenvalues ascending of the host matrix a (n x n column-major complex,
+ * Hermitised on the device), eigenvectors of the lowest k into vectors
+ * (n x k, may be NULL when k = 0). Orders above the one-launch reduction's
+ * limit take rocSOLVER zheevd. */
+int sdft_hip_hermitian_eigen(sdft_hip_context* ctx, size_t n, const double* a, size_t k, double* values,
+                             double* vectors, int* fused /* out, may be NULL: 1 = one-launch tridiagonalisation */);
 
 int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* params, const double* initial,
                       sdft_hip_davidson_result* result);

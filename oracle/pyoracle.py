@@ -199,3 +199,42 @@ def generalized_eigen(a, b):
     _check(lib().orc_hermitian_generalized_eigen(ct.c_long(n), _p(a), _p(b),
                                                  _p(w), _p(v)))
     return w, v
+
+
+def density_fft(hkl, coefficients, occupations, volume, support):
+    """plane_wave_density_fft (src/density_functionals.cpp:406-460) with
+    numpy's FFT on the reference's grid max(3, 2 m + 1) per axis:
+    rho(r) = sum_b occ_b / volume |backward FFT of c_b|^2 (unnormalised), and
+    the forward FFT / N at the support Miller triples."""
+    hkl = np.asarray(hkl, dtype=np.int64)
+    support = np.asarray(support, dtype=np.int64)
+    c = np.asarray(coefficients, dtype=np.complex128)
+    m = np.maximum(np.abs(support).max(axis=0), hkl.max(axis=0) - hkl.min(axis=0))
+    shape = tuple(int(max(3, 2 * v + 1)) for v in m)
+    count = int(np.prod(shape))
+    idx = tuple(np.mod(hkl[:, a], shape[a]) for a in range(3))
+    rho = np.zeros(shape)
+    for b in range(c.shape[1]):
+        if occupations[b] == 0:
+            continue
+        grid = np.zeros(shape, dtype=np.complex128)
+        grid[idx] = c[:, b]
+        psi = np.fft.ifftn(grid) * count  # unnormalised backward transform (e^{+i})
+        rho += occupations[b] / volume * (psi.real ** 2 + psi.imag ** 2)
+    f = np.fft.fftn(rho) / count
+    return f[tuple(np.mod(support[:, a], shape[a]) for a in range(3))]
+
+
+def density_brute_force(hkl, coefficients, occupations, volume, support):
+    """rho(G) = sum_b occ_b / volume sum_{G1 - G2 = G} c_b(G1) conj(c_b(G2))
+    (small bases only)."""
+    hkl = [tuple(int(v) for v in row) for row in np.asarray(hkl)]
+    c = np.asarray(coefficients, dtype=np.complex128)
+    acc = {}
+    for b in range(c.shape[1]):
+        w = occupations[b] / volume
+        for i, g1 in enumerate(hkl):
+            for j, g2 in enumerate(hkl):
+                key = (g1[0] - g2[0], g1[1] - g2[1], g1[2] - g2[2])
+                acc[key] = acc.get(key, 0.0) + w * c[i, b] * np.conj(c[j, b])
+    return np.array([acc.get(tuple(int(v) for v in k), 0.0) for k in np.asarray(support)])

@@ -68,8 +68,9 @@ def compensated_product(aop, a, b):
     return out
 
 
-def davidson(oracle_op, roots, max_iterations=256, tolerance=1e-12, initial=None):
-    """The reference Davidson driving the oracle operator through callbacks."""
+def davidson(oracle_op, roots, max_iterations=256, tolerance=1e-12, initial=None, band_group_size=0):
+    """The reference Davidson (or, with band_group_size > 0, the reference's
+    bounded band-group solver) driving the oracle operator through callbacks."""
     n = oracle_op.n
 
     def cb(user, which, x, cols, hy, sy):
@@ -93,5 +94,6 @@ def davidson(oracle_op, roots, max_iterations=256, tolerance=1e-12, initial=None
     _check(lib().ref_davidson_callbacks(ct.c_long(n), _p(oracle_op.h_diag), _p(oracle_op.s_diag),
                                         _p(oracle_op.kinetic), cbf, None, ct.c_long(roots),
                                         ct.c_long(max_iterations), ct.c_double(tolerance), _p(init), _p(ev),
-                                        _p(vec), _p(res), ct.byref(it), ct.byref(conv)))
+                                        _p(vec), _p(res), ct.byref(it), ct.byref(conv),
+                                        ct.c_long(band_group_size)))
     return dict(eigenvalues=ev, eigenvectors=vec, residual_norms=res, iterations=it.value, converged=bool(conv.value))

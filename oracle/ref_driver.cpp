@@ -71,7 +71,7 @@ typedef void (*ref_block_cb)(void* user, int which, const double* x, long cols, 
 int ref_davidson_callbacks(long n, const double* h_diag, const double* s_diag, const double* kinetic, ref_block_cb cb,
                            void* user, long roots, long max_iterations, double tolerance, const double* initial,
                            double* eigenvalues, double* eigenvectors, double* residuals, long* iterations,
-                           int* converged) {
+                           int* converged, long group_size) {
   try {
     static int pair_token = 0;
     sdft::HermitianOperator h, s;
@@ -99,8 +99,18 @@ int ref_davidson_callbacks(long n, const double* h_diag, const double* s_diag, c
     h.apply_pair_into = [block](const C* x, std::size_t c, C* hy, C* sy, bool) { block(2, x, c, hy, sy); };
     std::vector<C> init;
     if (initial) init.assign(reinterpret_cast<const C*>(initial), reinterpret_cast<const C*>(initial) + n * roots);
-    const auto r = sdft::solve_generalized_hermitian_davidson(h, s, std::size_t(roots), std::size_t(max_iterations),
-                                                              tolerance, init);
+    sdft::ComplexGeneralizedEigenResult r;
+    if (group_size > 0) {
+      // the reference's band-group entry reads its group size from the environment
+      const std::string g = std::to_string(group_size);
+      setenv("SDFT_BAND_GROUP_SIZE", g.c_str(), 1);
+      r = sdft::solve_generalized_hermitian_band_groups(h, s, std::size_t(roots), std::size_t(max_iterations),
+                                                        tolerance, init);
+      unsetenv("SDFT_BAND_GROUP_SIZE");
+    } else {
+      r = sdft::solve_generalized_hermitian_davidson(h, s, std::size_t(roots), std::size_t(max_iterations), tolerance,
+                                                     init);
+    }
     std::memcpy(eigenvalues, r.eigenvalues.data(), sizeof(double) * roots);
     std::memcpy(residuals, r.residual_norms.data(), sizeof(double) * roots);
     if (eigenvectors) std::memcpy(eigenvectors, r.eigenvectors.data(), sizeof(C) * n * roots);

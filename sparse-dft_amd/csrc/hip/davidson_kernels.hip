@@ -248,6 +248,68 @@ __global__ void k_finite_check(const double* __restrict__ x, long ld, long n, lo
   }
 }
 
+// max_ij |B_ij - delta_ij| over the m x m block (bits of a non-negative double,
+// folded with an unsigned max so that any NaN dominates); E = B - I when e != 0.
+__global__ __launch_bounds__(TPB) void k_identity_defect(const double* __restrict__ b, long ldb, long m,
+                                                         double* __restrict__ e, long lde,
+                                                         unsigned long long* __restrict__ max_bits) {
+  __shared__ double red[TPB];
+  double worst = 0.;
+  const long total = m * m;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const long i = t % m, j = t / m;
+    const double re = b[2 * (i + j * ldb)] - (i == j ? 1. : 0.), im = b[2 * (i + j * ldb) + 1];
+    if (e) {
+      e[2 * (i + j * lde)] = re;
+      e[2 * (i + j * lde) + 1] = im;
+    }
+    const double a = sqrt(re * re + im * im);
+    worst = (a > worst || a != a) ? a : worst;
+  }
+  red[threadIdx.x] = worst;
+  __syncthreads();
+  for (int w = TPB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      const double o = red[threadIdx.x + w];
+      if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(max_bits, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+// M = I - E/2 + 3 E2/8 (E2 = E E): the series of (I + E)^(-1/2) to second order
+__global__ void k_series_metric(const double* __restrict__ e, const double* __restrict__ e2, long ld, long m,
+                                double* __restrict__ out, long ldo) {
+  const long total = m * m;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const long i = t % m, j = t / m;
+    out[2 * (i + j * ldo)] = (i == j ? 1. : 0.) - 0.5 * e[2 * (i + j * ld)] + 0.375 * e2[2 * (i + j * ld)];
+    out[2 * (i + j * ldo) + 1] = -0.5 * e[2 * (i + j * ld) + 1] + 0.375 * e2[2 * (i + j * ld) + 1];
+  }
+}
+
+// out[i] = Re a_ii, out[m + i] = Re b_ii
+__global__ void k_diagonals(const double* __restrict__ a, long lda, const double* __restrict__ b, long ldb, long m,
+                            double* __restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < m; i += (long)gridDim.x * blockDim.x) {
+    out[i] = a[2 * (i + i * lda)];
+    out[m + i] = b[2 * (i + i * ldb)];
+  }
+}
+
+// zero the strictly lower triangle of an m x m block
+__global__ void k_zero_lower(double* __restrict__ a, long lda, long m) {
+  const long total = m * m;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const long i = t % m, j = t / m;
+    if (i > j) {
+      a[2 * (i + j * lda)] = 0.;
+      a[2 * (i + j * lda) + 1] = 0.;
+    }
+  }
+}
+
 dim3 grid_cols(long n, long cols) {
   long bx = (n + 255) / 256;
   if (bx > 64) bx = 64;
@@ -323,6 +385,33 @@ hipError_t launch_inv_sqrt_columns(double* u, long ld, long rows, long cols, con
 hipError_t launch_finite_check(const double* x, long ld, long n, long cols, int* bad, hipStream_t s) {
   if (cols <= 0) return hipSuccess;
   SDFT_CHECKED(hipLaunchKernelGGL(k_finite_check, dim3(512), dim3(256), 0, s, x, ld, n, cols, bad));
+}
+
+namespace {
+int square_blocks(long m) {
+  const long total = m * m;
+  return int(total / 256 + 1 > 2048 ? 2048 : total / 256 + 1);
+}
+}  // namespace
+hipError_t launch_identity_defect(const double* b, long ldb, long m, double* e, long lde,
+                                  unsigned long long* max_bits, hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_identity_defect, dim3(square_blocks(m)), dim3(TPB), 0, s, b, ldb, m, e, lde,
+                                  max_bits));
+}
+hipError_t launch_series_metric(const double* e, const double* e2, long ld, long m, double* out, long ldo,
+                                hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_series_metric, dim3(square_blocks(m)), dim3(256), 0, s, e, e2, ld, m, out, ldo));
+}
+hipError_t launch_diagonals(const double* a, long lda, const double* b, long ldb, long m, double* out,
+                            hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_diagonals, dim3(int((m + 255) / 256)), dim3(256), 0, s, a, lda, b, ldb, m, out));
+}
+hipError_t launch_zero_lower(double* a, long lda, long m, hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_zero_lower, dim3(square_blocks(m)), dim3(256), 0, s, a, lda, m));
 }
 
 }  // namespace sdft_hip

@@ -26,5 +26,16 @@ hipError_t launch_polish_update(const double* x, const double* d, long ld, long 
 hipError_t launch_hermitise(const double* m, long ldm, long size, double* out, long ldo, hipStream_t s);
 hipError_t launch_inv_sqrt_columns(double* u, long ld, long rows, long cols, const double* lambda, hipStream_t s);
 hipError_t launch_finite_check(const double* x, long ld, long n, long cols, int* bad, hipStream_t s);
+// max |B - I| (as the bits of a non-negative double, max-folded into *max_bits,
+// which the caller zeroes); E = B - I written when e is not null
+hipError_t launch_identity_defect(const double* b, long ldb, long m, double* e, long lde,
+                                  unsigned long long* max_bits, hipStream_t s);
+// out = I - E/2 + 3 E2/8
+hipError_t launch_series_metric(const double* e, const double* e2, long ld, long m, double* out, long ldo,
+                                hipStream_t s);
+// out[0:m] = Re diag(a), out[m:2m] = Re diag(b)
+hipError_t launch_diagonals(const double* a, long lda, const double* b, long ldb, long m, double* out,
+                            hipStream_t s);
+hipError_t launch_zero_lower(double* a, long lda, long m, hipStream_t s);
 
 }  // namespace sdft_hip

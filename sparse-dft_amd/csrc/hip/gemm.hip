@@ -16,7 +16,7 @@
 namespace sdft_hip {
 
 namespace {
-constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
+constexpr int BK = 16, PAD = 4;
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -43,80 +43,118 @@ __device__ __forceinline__ void load2(const S* p, double& re, double& im) {
   im = double(p[1]);
 }
 
-// One 64x64 output tile per workgroup (4 waves, 2x2 of 32x32), K range of
-// this split [k0, k1).
-template <typename T, typename TA, typename TB, bool ADJ_A, bool ADJ_B>
+// One BM x BN output tile per 256-thread workgroup (4 waves, WGM along M),
+// K range [k0, k1) of this split. Each wave computes (BM/WGM) x (BN/(4/WGM))
+// as 16 x 16 blocks; per 4-deep K step it first issues the independent
+// Ar*Br / Ar*Bi products of every block, then the dependent -Ai*Bi / Ai*Br
+// ones, so that no MFMA waits on the accumulator of the one just issued.
+// PF: the next K slab is loaded into registers while the current one is
+// multiplied from LDS.
+template <typename T, typename TA, typename TB, bool ADJ_A, bool ADJ_B, int TBM, int TBN, int WGM, bool PF>
 __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, const TA* __restrict__ A, long lda,
                                                const TB* __restrict__ B, long ldb, double* __restrict__ C, long ldc,
                                                const double* __restrict__ D, long ldd, double alpha, double beta,
                                                int kps, double* __restrict__ work) {
-  __shared__ T as_re[BK][BM + PAD], as_im[BK][BM + PAD];
-  __shared__ T bs_re[BK][BN + PAD], bs_im[BK][BN + PAD];
+  constexpr int WGN = 4 / WGM, WM = TBM / WGM, WN = TBN / WGN, IM = WM / 16, JN = WN / 16;
+  constexpr int EA = TBM * BK / 256, EB = TBN * BK / 256;
+  static_assert(EA * 256 == TBM * BK && EB * 256 == TBN * BK, "staging must tile the workgroup");
+  __shared__ T as_re[BK][TBM + PAD], as_im[BK][TBM + PAD];
+  __shared__ T bs_re[BK][TBN + PAD], bs_im[BK][TBN + PAD];
   using V = typename Acc<T>::type;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int m0 = blockIdx.x * TBM, n0 = blockIdx.y * TBN;
   const int k0 = blockIdx.z * kps, k1 = min(K, k0 + kps);
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  V acc_re[2][2], acc_im[2][2];
+  const int wm = (wave % WGM) * WM, wn = (wave / WGM) * WN;
+  V acc_re[IM][JN], acc_im[IM][JN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < IM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < JN; ++j) {
       acc_re[i][j] = V{0, 0, 0, 0};
       acc_im[i][j] = V{0, 0, 0, 0};
     }
-  for (int kb = k0; kb < k1; kb += BK) {
-    // stage A tile: A_eff[m][k], m in [m0, m0+BM), k in [kb, kb+BK)
+  double pa[EA][2], pb[EB][2];
+  // A_eff[m][k] and B_eff[k][n] of the K slab at kb into registers
+  auto load = [&](int kb) {
 #pragma unroll
-    for (int it = 0; it < (BM * BK) / 256; ++it) {
+    for (int it = 0; it < EA; ++it) {
       const int e = tid + it * 256;
       int mm, kk;
-      if (!ADJ_A) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
+      if (!ADJ_A) { mm = e % TBM; kk = e / TBM; } else { kk = e % BK; mm = e / BK; }
       const int m = m0 + mm, k = kb + kk;
       double re = 0, im = 0;
       if (m < M && k < k1) {
         if (!ADJ_A) load2(A + 2 * (m + (long)k * lda), re, im);
         else { load2(A + 2 * (k + (long)m * lda), re, im); im = -im; }
       }
-      as_re[kk][mm] = T(re);
-      as_im[kk][mm] = T(im);
+      pa[it][0] = re;
+      pa[it][1] = im;
     }
 #pragma unroll
-    for (int it = 0; it < (BN * BK) / 256; ++it) {
+    for (int it = 0; it < EB; ++it) {
       const int e = tid + it * 256;
       int nn, kk;
-      if (!ADJ_B) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
+      if (!ADJ_B) { kk = e % BK; nn = e / BK; } else { nn = e % TBN; kk = e / TBN; }
       const int n = n0 + nn, k = kb + kk;
       double re = 0, im = 0;
       if (n < N && k < k1) {
         if (!ADJ_B) load2(B + 2 * (k + (long)n * ldb), re, im);
         else { load2(B + 2 * (n + (long)k * ldb), re, im); im = -im; }
       }
-      bs_re[kk][nn] = T(re);
-      bs_im[kk][nn] = T(im);
+      pb[it][0] = re;
+      pb[it][1] = im;
     }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int it = 0; it < EA; ++it) {
+      const int e = tid + it * 256;
+      int mm, kk;
+      if (!ADJ_A) { mm = e % TBM; kk = e / TBM; } else { kk = e % BK; mm = e / BK; }
+      as_re[kk][mm] = T(pa[it][0]);
+      as_im[kk][mm] = T(pa[it][1]);
+    }
+#pragma unroll
+    for (int it = 0; it < EB; ++it) {
+      const int e = tid + it * 256;
+      int nn, kk;
+      if (!ADJ_B) { kk = e % BK; nn = e / BK; } else { nn = e % TBN; kk = e / TBN; }
+      bs_re[kk][nn] = T(pb[it][0]);
+      bs_im[kk][nn] = T(pb[it][1]);
+    }
+  };
+  if (PF && k0 < k1) load(k0);
+  for (int kb = k0; kb < k1; kb += BK) {
+    if (!PF) load(kb);
+    store();
     __syncthreads();
+    if (PF && kb + BK < k1) load(kb + BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int kr = ks + (lane >> 4), c = lane & 15;
-      T ar[2], ai[2], br[2], bi[2];
+      T ar[IM], ai[IM], br[JN], bi[JN];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < IM; ++i) {
         ar[i] = as_re[kr][wm + 16 * i + c];
         ai[i] = as_im[kr][wm + 16 * i + c];
       }
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      for (int j = 0; j < JN; ++j) {
         br[j] = bs_re[kr][wn + 16 * j + c];
         bi[j] = bs_im[kr][wn + 16 * j + c];
       }
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < IM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < JN; ++j) {
           acc_re[i][j] = Acc<T>::mfma(ar[i], br[j], acc_re[i][j]);
-          acc_re[i][j] = Acc<T>::mfma(-ai[i], bi[j], acc_re[i][j]);
           acc_im[i][j] = Acc<T>::mfma(ar[i], bi[j], acc_im[i][j]);
+        }
+#pragma unroll
+      for (int i = 0; i < IM; ++i)
+#pragma unroll
+        for (int j = 0; j < JN; ++j) {
+          acc_re[i][j] = Acc<T>::mfma(-ai[i], bi[j], acc_re[i][j]);
           acc_im[i][j] = Acc<T>::mfma(ai[i], br[j], acc_im[i][j]);
         }
     }
@@ -127,9 +165,9 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, const TA* __
   //   f64 16x16x4: register r holds row 4*r + lane/16
   const bool split = work != nullptr;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < IM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < JN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = sizeof(T) == 8 ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r;
@@ -256,19 +294,67 @@ __global__ void k_atom_blocks_f32(int np, int columns, int atoms, const int* __r
   }
 }
 
-template <typename T, typename TA, typename TB>
-hipError_t dispatch_ops(int aop, int bop, dim3 grid, int M, int N, int K, const TA* A, long lda, const TB* B, long ldb,
-                        double* C, long ldc, const double* D, long ldd, double alpha, double beta, int kps,
-                        double* work, hipStream_t s) {
-#define SDFT_LAUNCH(AA, BB) \
-  hipLaunchKernelGGL((k_zgemm<T, TA, TB, AA, BB>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, \
-                     alpha, beta, kps, work)
+// Tile configurations: 0 = 64 x 64 with prefetch, 1 = 64 x 64 without
+// (FP64 split-K slabs, where the measured prefetch variant is slower),
+// 2 = 128 x 32 with prefetch (tall, narrow outputs: rotations, back-projections).
+struct Tile {
+  int bm, bn;
+};
+constexpr Tile kTiles[3] = {{64, 64}, {64, 64}, {128, 32}};
+
+bool tall_shape(int M, int N) { return M >= 2048 && N <= 192 && long(M) >= 8L * N; }
+
+template <typename T, typename TA, typename TB, int CFG>
+hipError_t launch_cfg(int aop, int bop, dim3 grid, int M, int N, int K, const TA* A, long lda, const TB* B, long ldb,
+                      double* C, long ldc, const double* D, long ldd, double alpha, double beta, int kps,
+                      double* work, hipStream_t s) {
+  constexpr int TBM = kTiles[CFG].bm, TBN = kTiles[CFG].bn, WGM = CFG == 2 ? 4 : 2;
+  constexpr bool PF = CFG != 1;
+#define SDFT_LAUNCH(AA, BB)                                                                                      \
+  hipLaunchKernelGGL((k_zgemm<T, TA, TB, AA, BB, TBM, TBN, WGM, PF>), grid, dim3(256), 0, s, M, N, K, A, lda, B, \
+                     ldb, C, ldc, D, ldd, alpha, beta, kps, work)
   if (!aop && !bop) SDFT_LAUNCH(false, false);
   else if (aop && !bop) SDFT_LAUNCH(true, false);
   else if (!aop && bop) SDFT_LAUNCH(false, true);
   else SDFT_LAUNCH(true, true);
 #undef SDFT_LAUNCH
   return hipGetLastError();
+}
+
+template <typename T, typename TA, typename TB>
+hipError_t dispatch_ops(int cfg, int aop, int bop, dim3 grid, int M, int N, int K, const TA* A, long lda, const TB* B,
+                        long ldb, double* C, long ldc, const double* D, long ldd, double alpha, double beta, int kps,
+                        double* work, hipStream_t s, const LaunchEvents* ev) {
+  if (ev && ev->begin) {
+    const hipError_t e = hipEventRecord(ev->begin, s);
+    if (e != hipSuccess) return e;
+  }
+  hipError_t e;
+  if (cfg == 2) e = launch_cfg<T, TA, TB, 2>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work, s);
+  else if (cfg == 1 && sizeof(T) == 8)
+    e = launch_cfg<T, TA, TB, 1>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work, s);
+  else e = launch_cfg<T, TA, TB, 0>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work, s);
+  if (e != hipSuccess || !ev || !ev->end) return e;
+  return hipEventRecord(ev->end, s);
+}
+
+// Split-K plan of a shape: tile configuration, splits and K per split.
+struct Plan {
+  int cfg, splits, kps;
+  dim3 grid;
+};
+Plan plan_for(bool fp64, int M, int N, int K, bool have_work) {
+  Plan p{};
+  const bool tall = tall_shape(M, N);
+  const int bm = tall ? 128 : 64, bn = tall ? 32 : 64;
+  p.splits = zgemm_splits(M, N, K);
+  if (p.splits > 1 && !have_work) p.splits = 1;
+  p.kps = (K + p.splits - 1) / p.splits;
+  p.kps = ((p.kps + BK - 1) / BK) * BK;
+  p.splits = (K + p.kps - 1) / p.kps;
+  p.cfg = tall ? 2 : (fp64 && p.splits > 1 ? 1 : 0);
+  p.grid = dim3((M + bm - 1) / bm, (N + bn - 1) / bn, p.splits);
+  return p;
 }
 }  // namespace
 
@@ -278,7 +364,8 @@ std::size_t zgemm_workspace_bytes(int M, int N, int K) {
 }
 
 int zgemm_splits(int M, int N, int K) {
-  const long tiles = long((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+  const bool tall = tall_shape(M, N);
+  const long tiles = long((M + (tall ? 128 : 64) - 1) / (tall ? 128 : 64)) * ((N + (tall ? 32 : 64) - 1) / (tall ? 32 : 64));
   // aim for >= 512 workgroups on 256 CUs while keeping >= 256 K per split
   int splits = 1;
   while (tiles * splits < 512 && long(K) / (2 * splits) >= 256 && splits < 64) splits *= 2;
@@ -287,46 +374,38 @@ int zgemm_splits(int M, int N, int K) {
 
 hipError_t zgemm(Precision prec, int aop, int bop, int M, int N, int K, double alpha, const double* A, long lda,
                  const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta, double* work,
-                 std::size_t work_bytes, hipStream_t s) {
+                 std::size_t work_bytes, hipStream_t s, const LaunchEvents* ev) {
   if (M <= 0 || N <= 0) return hipSuccess;
   if (K <= 0) {
     // C = beta * D
     return launch_zaxpby(M, N, 0.0, nullptr, 0, beta, D, ldd, C, ldc, s);
   }
-  int splits = zgemm_splits(M, N, K);
-  if (splits > 1 && (work == nullptr || work_bytes < std::size_t(splits) * M * N * 16)) splits = 1;
-  int kps = (K + splits - 1) / splits;
-  kps = ((kps + BK - 1) / BK) * BK;
-  splits = (K + kps - 1) / kps;
-  const dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, splits);
-  double* wk = splits > 1 ? work : nullptr;
+  const bool room = work != nullptr && work_bytes >= zgemm_workspace_bytes(M, N, K);
+  const Plan pl = plan_for(prec == Precision::FP64, M, N, K, room);
+  double* wk = pl.splits > 1 ? work : nullptr;
   hipError_t e;
   if (prec == Precision::FP64)
-    e = dispatch_ops<double, double, double>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps,
-                                             wk, s);
+    e = dispatch_ops<double, double, double>(pl.cfg, aop, bop, pl.grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha,
+                                             beta, pl.kps, wk, s, ev);
   else
-    e = dispatch_ops<float, double, double>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps,
-                                            wk, s);
-  if (e != hipSuccess || splits == 1) return e;
-  hipLaunchKernelGGL(k_splitk_reduce, dim3(1024), dim3(256), 0, s, M, N, splits, work, C, ldc, D, ldd, beta);
+    e = dispatch_ops<float, double, double>(pl.cfg, aop, bop, pl.grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha,
+                                            beta, pl.kps, wk, s, ev);
+  if (e != hipSuccess || pl.splits == 1) return e;
+  hipLaunchKernelGGL(k_splitk_reduce, dim3(1024), dim3(256), 0, s, M, N, pl.splits, work, C, ldc, D, ldd, beta);
   return hipGetLastError();
 }
 
 hipError_t zgemm_f32_storage(int aop, int bop, int M, int N, int K, double alpha, const float* A, long lda,
                              const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta,
-                             double* work, std::size_t work_bytes, hipStream_t s) {
+                             double* work, std::size_t work_bytes, hipStream_t s, const LaunchEvents* ev) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
-  int splits = zgemm_splits(M, N, K);
-  if (splits > 1 && (work == nullptr || work_bytes < std::size_t(splits) * M * N * 16)) splits = 1;
-  int kps = (K + splits - 1) / splits;
-  kps = ((kps + BK - 1) / BK) * BK;
-  splits = (K + kps - 1) / kps;
-  const dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, splits);
-  double* wk = splits > 1 ? work : nullptr;
-  hipError_t e = dispatch_ops<float, float, double>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha,
-                                                    beta, kps, wk, s);
-  if (e != hipSuccess || splits == 1) return e;
-  hipLaunchKernelGGL(k_splitk_reduce, dim3(1024), dim3(256), 0, s, M, N, splits, work, C, ldc, D, ldd, beta);
+  const bool room = work != nullptr && work_bytes >= zgemm_workspace_bytes(M, N, K);
+  const Plan pl = plan_for(false, M, N, K, room);
+  double* wk = pl.splits > 1 ? work : nullptr;
+  hipError_t e = dispatch_ops<float, float, double>(pl.cfg, aop, bop, pl.grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd,
+                                                    alpha, beta, pl.kps, wk, s, ev);
+  if (e != hipSuccess || pl.splits == 1) return e;
+  hipLaunchKernelGGL(k_splitk_reduce, dim3(1024), dim3(256), 0, s, M, N, pl.splits, work, C, ldc, D, ldd, beta);
   return hipGetLastError();
 }
 

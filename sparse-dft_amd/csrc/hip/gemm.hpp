@@ -9,6 +9,12 @@ namespace sdft_hip {
 
 enum class Precision { FP64 = 0, FP32 = 1 };
 
+// Optional events recorded on the launch stream immediately before and after
+// the MFMA kernel of one GEMM (the split-K reduction is outside the pair).
+struct LaunchEvents {
+  hipEvent_t begin = nullptr, end = nullptr;
+};
+
 int zgemm_splits(int M, int N, int K);
 std::size_t zgemm_workspace_bytes(int M, int N, int K);
 
@@ -16,11 +22,11 @@ std::size_t zgemm_workspace_bytes(int M, int N, int K);
 // alias C. `prec` selects the arithmetic (FP64 or FP32 MFMA); storage is FP64.
 hipError_t zgemm(Precision prec, int aop, int bop, int M, int N, int K, double alpha, const double* A, long lda,
                  const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta, double* work,
-                 std::size_t work_bytes, hipStream_t s);
+                 std::size_t work_bytes, hipStream_t s, const LaunchEvents* ev = nullptr);
 // Same with A stored in FP32 (resident search copy of the projectors).
 hipError_t zgemm_f32_storage(int aop, int bop, int M, int N, int K, double alpha, const float* A, long lda,
                              const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta,
-                             double* work, std::size_t work_bytes, hipStream_t s);
+                             double* work, std::size_t work_bytes, hipStream_t s, const LaunchEvents* ev = nullptr);
 
 // W(:, 0:m) = D_a P_a and W(:, m:2m) = Q_a P_a per atom block (which_mask bit
 // 0: D, bit 1: Q). FP64 uses the compensated scalar order of the reference.

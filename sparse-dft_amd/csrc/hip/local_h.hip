@@ -79,11 +79,16 @@ __global__ __launch_bounds__(256) void k_sticks_backward(const double* __restric
 
 // B: grid (n2 slots, columns). mode 0: local H (bwd, x V, fwd, keep sticks);
 // mode 1: backward only, whole plane stored in slot order (potential setup).
+// In mode 0 the y-transforms run only on the x-lines that hold sticks (the
+// backward input and the forward output are zero / unused elsewhere).
+constexpr int kPlaneThreads = 512;
 template <typename T>
-__global__ __launch_bounds__(256) void k_planes(const cx<T>* __restrict__ t1, cx<T>* __restrict__ t2,
-                                                const int* __restrict__ stick_xy, int nsticks, int n0, int n1, int n2,
-                                                const cx<T>* __restrict__ tw_x, const cx<T>* __restrict__ tw_y,
-                                                const T* __restrict__ v_slots, int mode) {
+__global__ __launch_bounds__(kPlaneThreads) void k_planes(const cx<T>* __restrict__ t1, cx<T>* __restrict__ t2,
+                                                          const int* __restrict__ stick_xy, int nsticks, int n0,
+                                                          int n1, int n2, const cx<T>* __restrict__ tw_x,
+                                                          const cx<T>* __restrict__ tw_y,
+                                                          const T* __restrict__ v_slots, int mode, int xs0, int xl0,
+                                                          int xs1, int xl1) {
   extern __shared__ unsigned char smem_raw[];
   cx<T>* plane = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -95,11 +100,12 @@ __global__ __launch_bounds__(256) void k_planes(const cx<T>* __restrict__ t1, cx
   const cx<T>* in = t1 + (col * n2 + zs) * (long)nsticks;
   for (int s = tid; s < nsticks; s += nt) plane[stick_xy[s]] = in[s];
   __syncthreads();
-  // y lines: one per x, contiguous elements
-  const LineGeom gy{n0, n1, 1, false};
+  // y lines: one per x, contiguous elements (two ranges of active x)
+  const LineGeom gy0{xl0, n1, 1, false}, gy1{xl1, n1, 1, false};
   // x lines: one per y, elements strided by n1
   const LineGeom gx{n1, 1, n1, true};
-  line_fft_dispatch<T, +1, true>(n1, plane, gy, tw_y, tid, nt);
+  if (xl0) line_fft_dispatch<T, +1, true>(n1, plane + xs0 * n1, gy0, tw_y, tid, nt);
+  if (xl1) line_fft_dispatch<T, +1, true>(n1, plane + xs1 * n1, gy1, tw_y, tid, nt);
   line_fft_dispatch<T, +1, true>(n0, plane, gx, tw_x, tid, nt);
   if (mode == 1) {
     cx<T>* out = t2 + (col * n2 + zs) * (long)np;
@@ -114,7 +120,8 @@ __global__ __launch_bounds__(256) void k_planes(const cx<T>* __restrict__ t1, cx
   }
   __syncthreads();
   line_fft_dispatch<T, -1, false>(n0, plane, gx, tw_x, tid, nt);
-  line_fft_dispatch<T, -1, false>(n1, plane, gy, tw_y, tid, nt);
+  if (xl0) line_fft_dispatch<T, -1, false>(n1, plane + xs0 * n1, gy0, tw_y, tid, nt);
+  if (xl1) line_fft_dispatch<T, -1, false>(n1, plane + xs1 * n1, gy1, tw_y, tid, nt);
   cx<T>* out = t2 + (col * n2 + zs) * (long)nsticks;
   for (int s = tid; s < nsticks; s += nt) out[s] = plane[stick_xy[s]];
 }
@@ -254,9 +261,9 @@ hipError_t launch_local_h(const GridPlanDev& p, const double* x, long ldx, long 
   hipLaunchKernelGGL(k_sticks_backward<T>, ga, dim3(256), lds_lines, stream, x, ldx, p.stick_g, p.nsticks, p.n2, spb,
                      twz, reinterpret_cast<cx<T>*>(t1));
   const size_t lds_plane = size_t(p.n0) * p.n1 * sizeof(cx<T>);
-  hipLaunchKernelGGL(k_planes<T>, dim3(p.n2, (unsigned)columns), dim3(256), lds_plane, stream,
+  hipLaunchKernelGGL(k_planes<T>, dim3(p.n2, (unsigned)columns), dim3(kPlaneThreads), lds_plane, stream,
                      reinterpret_cast<const cx<T>*>(t1), reinterpret_cast<cx<T>*>(t2), p.stick_xy, p.nsticks, p.n0,
-                     p.n1, p.n2, twx, twy, v, 0);
+                     p.n1, p.n2, twx, twy, v, 0, p.xstart[0], p.xlines[0], p.xstart[1], p.xlines[1]);
   hipLaunchKernelGGL(k_sticks_forward<T>, ga, dim3(256), lds_lines, stream, reinterpret_cast<const cx<T>*>(t2),
                      p.stick_g, p.nsticks, p.n2, spb, twz, x, ldx, kinetic, y, ldy, 0.0, double(p.count));
   return hipGetLastError();
@@ -285,12 +292,103 @@ hipError_t launch_potential_setup(const GridPlanDev& full, const double* grid, v
   hipLaunchKernelGGL(k_sticks_backward<double>, ga, dim3(256), lds_lines, stream, grid, 0L, full.stick_g, full.nsticks,
                      full.n2, spb, reinterpret_cast<const cx<double>*>(full.tw64[2]), reinterpret_cast<cx<double>*>(t1));
   const size_t lds_plane = size_t(full.n0) * full.n1 * sizeof(cx<double>);
-  hipLaunchKernelGGL(k_planes<double>, dim3(full.n2, 1), dim3(256), lds_plane, stream,
+  hipLaunchKernelGGL(k_planes<double>, dim3(full.n2, 1), dim3(kPlaneThreads), lds_plane, stream,
                      reinterpret_cast<const cx<double>*>(t1), reinterpret_cast<cx<double>*>(t2), full.stick_xy,
                      full.nsticks, full.n0, full.n1, full.n2, reinterpret_cast<const cx<double>*>(full.tw64[0]),
-                     reinterpret_cast<const cx<double>*>(full.tw64[1]), (const double*)nullptr, 1);
+                     reinterpret_cast<const cx<double>*>(full.tw64[1]), (const double*)nullptr, 1, 0, full.n0, 0, 0);
   hipLaunchKernelGGL(k_potential_finish, dim3(ratio_blocks), dim3(256), 0, stream,
                      reinterpret_cast<const cx<double>*>(t2), (long)full.count, v64, v32, block_ratio);
+  return hipGetLastError();
+}
+
+// ---- smooth density (src/density_functionals.cpp:406-460)
+// density[i] += sum_c w_c |psi_c(i)|^2 over the columns of one chunk, in
+// column order (deterministic); planes: columns x count complex, slot order.
+__global__ void k_density_accumulate(const cx<double>* __restrict__ planes, long count, int columns,
+                                     const double* __restrict__ weights, double* __restrict__ density) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x) {
+    double acc = density[i];
+    for (int c = 0; c < columns; ++c) {
+      const cx<double> v = planes[(long)c * count + i];
+      acc += weights[c] * (v.x * v.x + v.y * v.y);
+    }
+    density[i] = acc;
+  }
+}
+
+// natural-order complex grid (re = slot-order real field, im = 0)
+__global__ void k_slots_to_natural_complex(const double* __restrict__ v_slots, int n0, int n1, int n2, int a0, int a1,
+                                           int a2, double* __restrict__ grid) {
+  const long count = (long)n0 * n1 * n2;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x) {
+    const int ys = int(i % n1);
+    const int xs = int((i / n1) % n0);
+    const int zs = int(i / ((long)n0 * n1));
+    const int x = slot_position(xs, a0, n0 / a0);
+    const int y = slot_position(ys, a1, n1 / a1);
+    const int z = slot_position(zs, a2, n2 / a2);
+    const long nat = ((long)x * n1 + y) * n2 + z;
+    grid[2 * nat] = v_slots[i];
+    grid[2 * nat + 1] = 0.0;
+  }
+}
+
+// out[j] = conj(B[G_j]) / count, B the backward transform of a real field in
+// slot order: the forward transform at G_j (real input).
+__global__ void k_gather_support(const cx<double>* __restrict__ planes, int n0, int n1, int n2, int a0, int a1,
+                                 int a2, const int* __restrict__ hkl, long nsupport, double inv_count,
+                                 double* __restrict__ out) {
+  for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < nsupport; j += (long)gridDim.x * blockDim.x) {
+    int x = hkl[3 * j] % n0, y = hkl[3 * j + 1] % n1, z = hkl[3 * j + 2] % n2;
+    x += x < 0 ? n0 : 0;
+    y += y < 0 ? n1 : 0;
+    z += z < 0 ? n2 : 0;
+    // slot of natural position p on an axis L = A * (L / A): p = b + (L / A) a -> s = a + A b
+    const int b0 = n0 / a0, b1 = n1 / a1, b2 = n2 / a2;
+    const int xs = x / b0 + a0 * (x % b0), ys = y / b1 + a1 * (y % b1), zs = z / b2 + a2 * (z % b2);
+    const cx<double> v = planes[((long)zs * n0 + xs) * n1 + ys];
+    out[2 * j] = v.x * inv_count;
+    out[2 * j + 1] = -v.y * inv_count;
+  }
+}
+
+hipError_t launch_backward_planes(const GridPlanDev& p, const double* x, long ldx, long columns, void* t1, void* t2,
+                                  hipStream_t stream) {
+  if (columns <= 0) return hipSuccess;
+  if (size_t(p.n0) * p.n1 * sizeof(cx<double>) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
+  const int spb = sticks_per_block(p.n2);
+  const dim3 ga((p.nsticks + spb - 1) / spb, (unsigned)columns);
+  const size_t lds_lines = size_t(spb) * p.n2 * sizeof(cx<double>);
+  hipLaunchKernelGGL(k_sticks_backward<double>, ga, dim3(256), lds_lines, stream, x, ldx, p.stick_g, p.nsticks, p.n2,
+                     spb, reinterpret_cast<const cx<double>*>(p.tw64[2]), reinterpret_cast<cx<double>*>(t1));
+  const size_t lds_plane = size_t(p.n0) * p.n1 * sizeof(cx<double>);
+  hipLaunchKernelGGL(k_planes<double>, dim3(p.n2, (unsigned)columns), dim3(kPlaneThreads), lds_plane, stream,
+                     reinterpret_cast<const cx<double>*>(t1), reinterpret_cast<cx<double>*>(t2), p.stick_xy,
+                     p.nsticks, p.n0, p.n1, p.n2, reinterpret_cast<const cx<double>*>(p.tw64[0]),
+                     reinterpret_cast<const cx<double>*>(p.tw64[1]), (const double*)nullptr, 1, p.xstart[0],
+                     p.xlines[0], p.xstart[1], p.xlines[1]);
+  return hipGetLastError();
+}
+
+hipError_t launch_density_accumulate(const void* planes, long count, int columns, const double* weights,
+                                     double* density, hipStream_t stream) {
+  hipLaunchKernelGGL(k_density_accumulate, dim3(1024), dim3(256), 0, stream,
+                     reinterpret_cast<const cx<double>*>(planes), count, columns, weights, density);
+  return hipGetLastError();
+}
+
+hipError_t launch_slots_to_natural_complex(const GridPlanDev& p, const double* v_slots, double* grid,
+                                           hipStream_t stream) {
+  hipLaunchKernelGGL(k_slots_to_natural_complex, dim3(1024), dim3(256), 0, stream, v_slots, p.n0, p.n1, p.n2,
+                     p.split[0], p.split[1], p.split[2], grid);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_support(const GridPlanDev& p, const void* planes, const int* hkl, long nsupport,
+                                 double* out, hipStream_t stream) {
+  if (nsupport <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_support, dim3(256), dim3(256), 0, stream, reinterpret_cast<const cx<double>*>(planes),
+                     p.n0, p.n1, p.n2, p.split[0], p.split[1], p.split[2], hkl, nsupport, 1.0 / double(p.count), out);
   return hipGetLastError();
 }
 

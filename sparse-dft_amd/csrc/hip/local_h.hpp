@@ -15,6 +15,9 @@ struct GridPlanDev {
   const void* tw32[3] = {nullptr, nullptr, nullptr};  // per axis FP32
   const double* v64 = nullptr;    // V(r) in slot order [zs][xs][ys]
   const float* v32 = nullptr;
+  // x-lines that hold sticks, as two ranges [start, start + lines) of natural
+  // x: the y-transforms of the local H run on these lines only
+  int xstart[2] = {0, 0}, xlines[2] = {0, 0};
 };
 
 // y(:,c) = T x(:,c) + FFT_fwd(V . FFT_bwd(x(:,c))) / N for c < columns.
@@ -29,6 +32,22 @@ hipError_t launch_potential_setup(const GridPlanDev& full, const double* grid, v
                                   float* v32, double* block_ratio, int ratio_blocks, hipStream_t stream);
 hipError_t launch_potential_to_slots(const GridPlanDev& p, const double* v_nat, double* v64, float* v32,
                                      hipStream_t stream);
+// Largest dynamic LDS of one workgroup on the current device.
+int max_lds_bytes();
+
+// Smooth density pieces: backward transform of columns into full slot-order
+// planes (t2: columns x count complex), weighted |psi|^2 accumulation, the
+// natural-order complex copy of a slot-order real field, and the gather of
+// conj(backward transform) / count at Miller indices (the forward transform of
+// a real field).
+hipError_t launch_backward_planes(const GridPlanDev& p, const double* x, long ldx, long columns, void* t1, void* t2,
+                                  hipStream_t stream);
+hipError_t launch_density_accumulate(const void* planes, long count, int columns, const double* weights,
+                                     double* density, hipStream_t stream);
+hipError_t launch_slots_to_natural_complex(const GridPlanDev& p, const double* v_slots, double* grid,
+                                           hipStream_t stream);
+hipError_t launch_gather_support(const GridPlanDev& p, const void* planes, const int* hkl, long nsupport,
+                                 double* out, hipStream_t stream);
 hipError_t launch_potential_to_natural(const GridPlanDev& p, const double* v64, double* v_nat, hipStream_t stream);
 
 }  // namespace sdft_hip

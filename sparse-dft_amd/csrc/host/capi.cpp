@@ -12,9 +12,12 @@
 #include <string>
 #include <vector>
 
+#include "../hip/davidson_kernels.hpp"
 #include "davidson.hpp"
+#include "dense_eigen.hpp"
 #include "device.hpp"
 #include "pw_basis.hpp"
+#include "snapshot.hpp"
 
 struct sdft_hip_context {
   sdft_hip::Context impl;
@@ -23,6 +26,7 @@ struct sdft_hip_context {
 struct sdft_hip_operator {
   sdft_hip::DeviceOperator impl;
   sdft_hip_operator(sdft_hip::Context& c, const sdft_hip::OperatorDesc& d) : impl(c, d) {}
+  sdft_hip_operator(sdft_hip::Context& c, const sdft_hip::SnapshotArrays& s) : impl(c, s) {}
 };
 
 namespace {
@@ -107,6 +111,23 @@ int sdft_hip_device_name(sdft_hip_context* ctx, char* buffer, size_t size) {
 
 int sdft_hip_synchronize(sdft_hip_context* ctx) {
   return guarded([&] { ctx->impl.sync(); });
+}
+
+int sdft_hip_kernel_clock(sdft_hip_context* ctx, int enable) {
+  return guarded([&] {
+    if (!ctx) throw std::invalid_argument("HIP: null context");
+    ctx->impl.clock_enable(enable != 0);
+  });
+}
+
+int sdft_hip_kernel_clock_stats(sdft_hip_context* ctx, int kind, long* launches, double* milliseconds, double* work) {
+  return guarded([&] {
+    if (!ctx || !launches || !milliseconds || !work) throw std::invalid_argument("HIP: null kernel clock argument");
+    const auto st = ctx->impl.clock_stat(kind);
+    *launches = st.launches;
+    *milliseconds = st.milliseconds;
+    *work = st.work;
+  });
 }
 
 // src/compute_cuda.cu:186-209 semantics with MFMA kernels underneath
@@ -475,6 +496,7 @@ int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* par
     p.handoff_restart = params->handoff_restart;
     p.verbose = params->verbose;
     p.timing = params->timing;
+    p.band_group_size = params->band_group_size;
     if (p.search_fp32 != 0 && p.search_fp32 != 1) throw std::invalid_argument("HIP: search precision must be 0 or 1");
     const auto out = sdft_hip::davidson(op->impl, p, reinterpret_cast<const std::complex<double>*>(initial));
     std::copy(out.eigenvalues.begin(), out.eigenvalues.end(), result->eigenvalues);
@@ -491,6 +513,115 @@ int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* par
     result->seconds_gemm = out.stats.seconds_gemm;
     result->seconds_ritz = out.stats.seconds_ritz;
     result->seconds_ortho = out.stats.seconds_ortho;
+    result->cholesky_blocks = out.stats.cholesky_blocks;
+    result->rr_cholesky = out.stats.rr_cholesky;
+  });
+}
+
+int sdft_hip_operator_project(sdft_hip_operator* op, const double* x, size_t columns, double* p, int precision) {
+  return guarded([&] {
+    if (!op || !x || !p || !columns || (precision != 0 && precision != 1))
+      throw std::invalid_argument("HIP: invalid PAW projection buffers");
+    auto& o = op->impl;
+    hipStream_t s = o.context().stream();
+    const size_t n = o.n(), np = o.np();
+    sdft_hip::DevBuf<double> dx, dp;
+    dx.upload(x, 2 * n * columns, s);
+    dp.reserve(2 * std::max<size_t>(np, 1) * columns);
+    o.project(dx.p, long(columns), dp.p, precision ? sdft_hip::Precision::FP32 : sdft_hip::Precision::FP64);
+    sdft_hip::check(hipMemcpyAsync(p, dp.p, 16 * np * columns, hipMemcpyDeviceToHost, s), "download");
+    o.context().sync();
+  });
+}
+
+int sdft_hip_operator_projector_metric(sdft_hip_operator* op, const double* p, size_t columns, double* qp) {
+  return guarded([&] {
+    if (!op || !p || !qp || !columns) throw std::invalid_argument("HIP: invalid projector metric buffers");
+    auto& o = op->impl;
+    hipStream_t s = o.context().stream();
+    const size_t np = o.np();
+    sdft_hip::DevBuf<double> dp, dq;
+    dp.upload(p, 2 * np * columns, s);
+    dq.reserve(2 * std::max<size_t>(np, 1) * columns);
+    o.projector_metric(dp.p, long(columns), dq.p);
+    sdft_hip::check(hipMemcpyAsync(qp, dq.p, 16 * np * columns, hipMemcpyDeviceToHost, s), "download");
+    o.context().sync();
+  });
+}
+
+int sdft_hip_operator_density(sdft_hip_operator* op, const double* coefficients, size_t bands,
+                              const double* occupations, double volume, size_t nsupport, const int* support_hkl,
+                              double* out) {
+  return guarded([&] {
+    if (!op || !coefficients || !occupations || !support_hkl || !out || !bands || !nsupport)
+      throw std::invalid_argument("invalid plane-wave FFT density dimensions");
+    auto& o = op->impl;
+    const size_t n = o.n();
+    for (size_t i = 0; i < 2 * n * bands; ++i)
+      if (!std::isfinite(coefficients[i])) throw std::invalid_argument("nonfinite wavefunction coefficient");
+    sdft_hip::DevBuf<double> dx;
+    dx.upload(coefficients, 2 * n * bands, o.context().stream());
+    const auto r = o.density(dx.p, long(bands), occupations, volume, support_hkl, long(nsupport));
+    std::memcpy(out, r.data(), r.size() * sizeof(r[0]));
+  });
+}
+
+int sdft_hip_snapshot_info_read(const char* path, sdft_hip_snapshot_info* info) {
+  return guarded([&] {
+    if (!path || !info) throw std::invalid_argument("HIP: invalid snapshot arguments");
+    const sdft_hip::SnapshotArrays s = sdft_hip::read_snapshot(path);
+    info->format = int(s.form);
+    info->n = s.n;
+    info->count = s.count;
+    for (int a = 0; a < 3; ++a) info->shape[a] = s.shape[a];
+    info->projectors = s.projectors;
+    info->atoms = s.channels.size();
+    info->roots = s.roots;
+    info->iterations = s.iterations;
+    info->initial_columns = s.initial.size() / s.n;
+    info->regularization = s.regularization;
+    info->tolerance = s.tolerance;
+  });
+}
+
+int sdft_hip_snapshot_load(sdft_hip_context* ctx, const char* path, double* initial, sdft_hip_operator** out) {
+  return guarded([&] {
+    if (!ctx || !path || !out) throw std::invalid_argument("HIP: invalid snapshot arguments");
+    const sdft_hip::SnapshotArrays s = sdft_hip::read_snapshot(path);
+    auto op = std::make_unique<sdft_hip_operator>(ctx->impl, s);
+    if (initial && !s.initial.empty()) std::memcpy(initial, s.initial.data(), s.initial.size() * sizeof(s.initial[0]));
+    *out = op.release();
+  });
+}
+
+int sdft_hip_snapshot_save(sdft_hip_operator* op, const char* path, size_t roots, size_t iterations,
+                           double tolerance, const double* initial) {
+  return guarded([&] {
+    if (!op || !path) throw std::invalid_argument("HIP: invalid snapshot arguments");
+    const auto s = op->impl.snapshot(roots, iterations, tolerance, reinterpret_cast<const std::complex<double>*>(initial));
+    sdft_hip::write_snapshot(path, s);
+  });
+}
+
+int sdft_hip_hermitian_eigen(sdft_hip_context* ctx, size_t n, const double* a, size_t k, double* values,
+                             double* vectors, int* fused) {
+  return guarded([&] {
+    if (!ctx || !a || !values || n == 0 || k > n || (k && !vectors))
+      throw std::invalid_argument("HIP: invalid Hermitian eigensolve arguments");
+    auto& cx = ctx->impl;
+    hipStream_t s = cx.stream();
+    sdft_hip::DevBuf<double> in, herm, vec;
+    in.reserve(2 * n * n);
+    herm.reserve(2 * n * n);
+    vec.reserve(2 * n * std::max<size_t>(k, 1));
+    sdft_hip::check(hipMemcpyAsync(in.p, a, 16 * n * n, hipMemcpyHostToDevice, s), "upload");
+    sdft_hip::check(sdft_hip::launch_hermitise(in.p, long(n), long(n), herm.p, long(n), s), "hermitise");
+    sdft_hip::HermitianEigensolver eig(cx, long(n));
+    const std::vector<double> w = eig.solve(herm.p, long(n), long(k), vec.p);
+    std::copy(w.begin(), w.end(), values);
+    if (k) sdft_hip::check(hipMemcpyAsync(vectors, vec.p, 16 * n * k, hipMemcpyDeviceToHost, s), "download");
+    cx.sync();
+    if (fused) *fused = eig.launches_fused() > 0 ? 1 : 0;
   });
 }
 

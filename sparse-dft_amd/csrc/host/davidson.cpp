@@ -12,6 +12,7 @@
 #include <numeric>
 
 #include "../hip/davidson_kernels.hpp"
+#include "dense_eigen.hpp"
 
 namespace sdft_hip {
 namespace {
@@ -27,6 +28,12 @@ class Solver {
     if (p.subspace_factor < 4 || p.subspace_factor > 16)
       throw std::invalid_argument("HIP: Davidson subspace factor must be in [4,16]");
     cap_ = std::min<long>(n_, std::max<long>(p.subspace_factor * R_, 64));
+    if (p.capacity) {
+      // src/pw_core.cpp:733-734
+      if (p.capacity < std::min<long>(n_, 2 * R_))
+        throw std::invalid_argument("HIP: Davidson local subspace must hold roots and guard states");
+      cap_ = std::min<long>(n_, p.capacity);
+    }
     exact_ = !p.search_fp32;
     basis_.reserve(2 * n_ * cap_);
     hb_.reserve(2 * n_ * cap_);
@@ -52,6 +59,11 @@ class Solver {
     qbuf_.reserve(2 * cap_ * cap_ + 16);
     corr_.reserve(2 * n_ * R_);
     retained_.reserve(2 * n_ * cap_);
+    eig_ = std::make_unique<HermitianEigensolver>(ctx_, cap_);
+    if (p.excluded_count < 0 || (p.excluded_count && !p.excluded))
+      throw std::invalid_argument("invalid completed-band dimensions");
+    excl_ = p.excluded_count;
+    if (excl_) setup_excluded(p.excluded);
   }
   DavidsonOutput run(const C* initial);
 
@@ -110,17 +122,58 @@ class Solver {
   }
   // Hermitian eigen-decomposition of a device matrix (overwritten by vectors)
   std::vector<double> heevd(double* a, long m, long lda) {
+    const double t0 = p_.timing ? (sync(), now()) : 0;
+    std::vector<double> w = heevd_device(a, m, lda);
+    if (p_.timing) stats_.seconds_ritz += now() - t0;
+    return w;
+  }
+  std::vector<double> heevd_device(double* a, long m, long lda) {
     double* w = small_.p + 8;
     double* e = w + m;
     int* info = reinterpret_cast<int*>(small_.p);
-    const double t0 = p_.timing ? (sync(), now()) : 0;
     check(rocsolver_zheevd(ctx_.blas(), rocblas_evect_original, rocblas_fill_lower, rocblas_int(m),
                            reinterpret_cast<rocblas_double_complex*>(a), rocblas_int(lda), w, e, info),
           "rocsolver_zheevd");
     std::vector<double> h = to_host(small_.p, 8 + m);
-    if (p_.timing) stats_.seconds_ritz += now() - t0;
     if (*reinterpret_cast<int*>(h.data())) throw std::runtime_error("HIP: Hermitian eigensolve failed");
     return std::vector<double>(h.begin() + 8, h.begin() + 8 + m);
+  }
+  // max_ij |B_ij - delta_ij| of an m x m device block; E = B - I into e if given
+  double identity_defect(const double* b, long m, long ld, double* e) {
+    auto* bits = reinterpret_cast<unsigned long long*>(small_.p);
+    check(hipMemsetAsync(bits, 0, sizeof(unsigned long long), s_), "memset");
+    check(launch_identity_defect(b, ld, m, e, m, bits, s_), "identity defect");
+    return to_host(small_.p, 1)[0];  // the bits of a non-negative double (or NaN)
+  }
+  // Rayleigh-Ritz of the projected pencil (A, B) of the S-orthonormal basis,
+  // B = I + E: congruence with M = I - E/2 + 3E^2/8, so that M B M = I + O(E^3),
+  // then the Hermitian eigensolve of M A M and vectors M Y. This solves the
+  // same pencil as zhegvd (src/pw_core.cpp:104-122) to O(|E|^3); a basis whose
+  // metric is further from the identity than 1e-4 takes the Cholesky reduction.
+  // A returns the first `vectors` B-orthonormal eigenvectors, ascending.
+  std::vector<double> rr_solve(double* a, double* b, long m, long vectors) {
+    const double t0 = p_.timing ? (sync(), now()) : 0;
+    double* e = m3_.p;
+    double* e2 = m3_.p + 2 * m * m;
+    double* mm = qbuf_.p;
+    double* t = m2_.p;
+    const double defect = identity_defect(b, m, m, e);
+    if (!(defect <= 1e-4)) {
+      ++stats_.rr_cholesky;
+      if (p_.timing) stats_.seconds_ritz += now() - t0;
+      return hegvd(a, b, m, m);
+    }
+    const Precision f64 = Precision::FP64;
+    ctx_.zgemm(f64, 0, 0, m, m, m, 1.0, e, m, e, m, e2, m, nullptr, 0, 0.0);
+    check(launch_series_metric(e, e2, m, m, mm, m, s_), "series metric");
+    ctx_.zgemm(f64, 0, 0, m, m, m, 1.0, mm, m, a, m, t, m, nullptr, 0, 0.0);
+    ctx_.zgemm(f64, 0, 0, m, m, m, 1.0, t, m, mm, m, e2, m, nullptr, 0, 0.0);
+    check(launch_hermitise(e2, m, m, e, m, s_), "hermitise");
+    const long k = std::min(vectors, m);
+    std::vector<double> w = eig_->solve(e, m, k, t);
+    ctx_.zgemm(f64, 0, 0, m, k, m, 1.0, mm, m, t, m, a, m, nullptr, 0, 0.0);
+    if (p_.timing) { sync(); stats_.seconds_ritz += now() - t0; }
+    return w;
   }
   // Generalized A v = l B v (itype 1), A overwritten by B-orthonormal vectors:
   // B = L L^H (potrf), C = L^{-1} A L^{-H} (two BLAS3 triangular solves),
@@ -163,6 +216,11 @@ class Solver {
 
   bool append(const double* v_in);
   bool append_block(double* v, long cols);
+  // completed bands E (S-orthonormal) and their S images; v -= E (SE)^H v
+  void setup_excluded(const double* e);
+  void remove_excluded(double* v, long cols);
+  bool cholesky_normalise(const double* v, const double* metric, const double* gram, double* gh, long cols,
+                          bool exact);
   void update_projected(const double* applied, double* cached);
   void flush_h();
   void certify();
@@ -190,6 +248,10 @@ class Solver {
   bool exact_ = true, incremental_ = true, converged_ = false;
   DevBuf<double> basis_, hb_, sb_, ch_, cs_, x_, sx_, hx_, res_;
   DevBuf<double> small_, w1_, w2_, w3_, w4_, m1_, m2_, m3_, rr_, rrb_, qbuf_, corr_, retained_;
+  std::unique_ptr<HermitianEigensolver> eig_;
+  long excl_ = 0;
+  const double* excluded_ = nullptr;
+  DevBuf<double> excluded_s_, excl_coef_;
   long size_ = 0, computed_h_ = 0, projected_size_ = 0;
   std::vector<double> values_, res_norms_;
   DavidsonStats stats_;
@@ -208,7 +270,8 @@ bool Solver::append(const double* v_in) {
   check(launch_divide_columns(v, n_, n_, 1, small_.p, s_), "scale");
   apply(1, v, 1, nullptr, sv, true);
   for (int pass = 0; pass < 2; ++pass) {
-    if (pass) apply(1, v, 1, nullptr, sv, true);
+    if (excl_) remove_excluded(v, 1);  // src/pw_core.cpp:810
+    if (pass || excl_) apply(1, v, 1, nullptr, sv, true);
     if (size_ < 8) {
       for (long j = 0; j < size_; ++j) {
         double* proj = m1_.p;
@@ -260,15 +323,24 @@ bool Solver::append_block(double* v, long cols) {
   double* metric = w4_.p;
   copy(original, v, n_ * cols);
   apply(1, v, cols, nullptr, metric, exact);
-  for (int pass = 0; size_ && pass < 2; ++pass) {
-    double* proj = m2_.p;
-    gemm(exact, 1, 0, size_, cols, n_, 1.0, basis_.p, n_, metric, n_, proj, size_);
-    gemm(exact, 0, 0, n_, cols, size_, -1.0, basis_.p, n_, proj, size_, v, n_, v, n_, 1.0);
+  for (int pass = 0; (size_ || excl_) && pass < 2; ++pass) {
+    if (size_) {
+      double* proj = m2_.p;
+      gemm(exact, 1, 0, size_, cols, n_, 1.0, basis_.p, n_, metric, n_, proj, size_);
+      gemm(exact, 0, 0, n_, cols, size_, -1.0, basis_.p, n_, proj, size_, v, n_, v, n_, 1.0);
+    }
+    remove_excluded(v, cols);  // src/pw_core.cpp:909-922
     apply(1, v, cols, nullptr, metric, exact);
   }
   double* gram = m3_.p;
   double* gh = m3_.p + 2 * cols * cols;
   gemm(exact, 1, 0, cols, cols, n_, 1.0, v, n_, metric, n_, gram, cols);
+  check(launch_hermitise(gram, cols, cols, gh, cols, s_), "hermitise");
+  if (cholesky_normalise(v, metric, gram, gh, cols, exact)) {
+    size_ += cols;
+    if (p_.timing) { sync(); stats_.seconds_ortho += now() - t0; }
+    return true;
+  }
   check(launch_hermitise(gram, cols, cols, gh, cols, s_), "hermitise");
   const std::vector<double> lam = heevd(gh, cols, cols);
   const double largest = *std::max_element(lam.begin(), lam.end());
@@ -285,6 +357,65 @@ bool Solver::append_block(double* v, long cols) {
   gemm(exact, 0, 0, n_, cols, cols, 1.0, metric, n_, gh, cols, col(sb_, size_, n_), n_);
   size_ += cols;
   if (p_.timing) { sync(); stats_.seconds_ortho += now() - t0; }
+  return true;
+}
+
+// src/pw_core.cpp:675-711: completed bands of a band-group local solve are
+// checked S-orthonormal and projected out of every new search direction.
+void Solver::setup_excluded(const double* e) {
+  excluded_ = e;
+  excluded_s_.reserve(std::size_t(2 * n_ * excl_));
+  excl_coef_.reserve(std::size_t(2 * excl_ * std::max(cap_, R_)));
+  apply(1, e, excl_, nullptr, excluded_s_.p, true);
+  DevBuf<double> gram;
+  gram.reserve(std::size_t(2 * excl_ * excl_));
+  gemm(true, 1, 0, excl_, excl_, n_, 1.0, e, n_, excluded_s_.p, n_, gram.p, excl_);
+  if (!(identity_defect(gram.p, excl_, excl_, nullptr) <= 1e-10))
+    throw std::invalid_argument("completed bands must be S orthonormal");
+}
+
+void Solver::remove_excluded(double* v, long cols) {
+  if (!excl_ || cols <= 0) return;
+  gemm(true, 1, 0, excl_, cols, n_, 1.0, excluded_s_.p, n_, v, n_, excl_coef_.p, excl_);
+  gemm(true, 0, 0, n_, cols, excl_, -1.0, excluded_, n_, excl_coef_.p, excl_, v, n_, v, n_, 1.0);
+}
+
+// Cholesky form of the block normalisation: G = U^H U, W = V U^{-1} spans the
+// same space as the eigen-normalised block V U diag(l)^(-1/2) of the reference
+// (src/pw_core.cpp:960-1000), so the Galerkin pencil is unchanged. It is taken
+// only when the Cholesky pivots bound cond(G) by 1e8 and the result is
+// S-orthonormal to 1e-10 (FP64) or 1e-4 (FP32 search arithmetic); otherwise the
+// caller runs the reference's eigen-normalisation and its dependency test.
+// On success W and S W are written after the current basis.
+bool Solver::cholesky_normalise(const double* v, const double* metric, const double* gram, double* gh, long cols,
+                                bool exact) {
+  int* info = reinterpret_cast<int*>(small_.p);
+  auto* U = reinterpret_cast<rocblas_double_complex*>(gh);
+  check(rocsolver_zpotrf(ctx_.blas(), rocblas_fill_upper, rocblas_int(cols), U, rocblas_int(cols), info),
+        "rocsolver_zpotrf");
+  double* diag = small_.p + 8;
+  check(launch_diagonals(gram, cols, gh, cols, cols, diag, s_), "diagonals");
+  const std::vector<double> h = to_host(small_.p, 8 + 2 * cols);
+  if (*reinterpret_cast<const int*>(h.data())) return false;
+  double gmax = 0, umin = std::numeric_limits<double>::infinity();
+  for (long i = 0; i < cols; ++i) {
+    gmax = std::max(gmax, h[8 + i]);
+    umin = std::min(umin, h[8 + cols + i] * h[8 + cols + i]);
+  }
+  if (!(gmax > 0) || !(umin >= 1e-8 * gmax) || !std::isfinite(gmax)) return false;
+  check(rocsolver_ztrtri(ctx_.blas(), rocblas_fill_upper, rocblas_diagonal_non_unit, rocblas_int(cols), U,
+                         rocblas_int(cols), info),
+        "rocsolver_ztrtri");
+  check(launch_zero_lower(gh, cols, cols, s_), "zero lower");
+  double* w = col(basis_, size_, n_);
+  double* sw = col(sb_, size_, n_);
+  gemm(exact, 0, 0, n_, cols, cols, 1.0, v, n_, gh, cols, w, n_);
+  gemm(exact, 0, 0, n_, cols, cols, 1.0, metric, n_, gh, cols, sw, n_);
+  double* g2 = m2_.p;
+  gemm(exact, 1, 0, cols, cols, n_, 1.0, w, n_, sw, n_, g2, cols);
+  const double defect = identity_defect(g2, cols, cols, nullptr);
+  if (!(defect <= (exact ? 1e-10 : 1e-4))) return false;
+  ++stats_.cholesky_blocks;
   return true;
 }
 
@@ -535,7 +666,7 @@ DavidsonOutput Solver::run(const C* initial) {
     double* B = rrb.p;
     check(launch_hermitise(ch_.p, cap_, size_, A, size_, s_), "hermitise");
     check(launch_hermitise(cs_.p, cap_, size_, B, size_, s_), "hermitise");
-    const std::vector<double> w = hegvd(A, B, size_, size_);
+    const std::vector<double> w = rr_solve(A, B, size_, std::min(size_, 2 * R));
     for (long r = 0; r < R; ++r) values_[r] = w[r];
     // Ritz vectors and residuals (src/pw_core.cpp:1169-1183)
     gemm(exact_, 0, 0, n, R, size_, 1.0, basis_.p, n, A, size_, x_.p, n);
@@ -587,7 +718,10 @@ DavidsonOutput Solver::run(const C* initial) {
       }
     }
     bool polished = false;
-    if (!converged_ && tol_ <= 1e-12 && max_res() <= 1e-10 && max_res() < .5 * strict_entry &&
+    // a capped local band group is not a complete invariant manifold: no
+    // strict polish there (src/pw_core.cpp:1269-1273)
+    if (!(p_.band_group_local && p_.capacity) && !converged_ && tol_ <= 1e-12 && max_res() <= 1e-10 &&
+        max_res() < .5 * strict_entry &&
         (polish_attempts < 16 || manifold_attempts < 4)) {
       if (!fresh) certify();
       strict_entry = max_res();
@@ -682,8 +816,169 @@ DavidsonOutput Solver::run(const C* initial) {
 }  // namespace
 
 DavidsonOutput davidson(DeviceOperator& op, const DavidsonParams& params, const std::complex<double>* initial) {
+  if (params.band_group_size > 0 && !params.band_group_local) return davidson_band_groups(op, params, initial);
   Solver s(op, params);
   return s.run(initial);
+}
+
+// src/band_group_solver.cpp:57-211 (default: two sweeps, each group solved to
+// the full iteration limit, no image reuse).
+DavidsonOutput davidson_band_groups(DeviceOperator& op, const DavidsonParams& params,
+                                    const std::complex<double>* initial) {
+  using C = std::complex<double>;
+  const long n = long(op.n()), R = params.roots, G = params.band_group_size;
+  if (!n || R <= 0 || R > n || params.max_iterations <= 0 || !std::isfinite(params.tolerance) ||
+      params.tolerance <= 0)
+    throw std::invalid_argument("invalid bounded band-group problem");
+  if (G < 1 || G > 64) throw std::invalid_argument("SDFT_BAND_GROUP_SIZE must be in [1,64]");
+  Context& ctx = op.context();
+  hipStream_t s = ctx.stream();
+  const double t_start = now();
+  auto sync = [&] { ctx.sync(); };
+  DevBuf<double> x, hx, sx, tmp, small, rot, hh, ss;
+  x.reserve(std::size_t(2 * n * R));
+  hx.reserve(std::size_t(2 * n * R));
+  sx.reserve(std::size_t(2 * n * R));
+  tmp.reserve(std::size_t(2 * n * R));
+  small.reserve(std::size_t(8 * R + 64));
+  rot.reserve(std::size_t(2 * R * R));
+  hh.reserve(std::size_t(2 * R * R));
+  ss.reserve(std::size_t(2 * R * R));
+  if (initial) {
+    check(hipMemcpyAsync(x.p, initial, 16 * std::size_t(n) * R, hipMemcpyHostToDevice, s), "upload");
+  } else {
+    check(hipMemsetAsync(x.p, 0, 16 * std::size_t(n) * R, s), "zero");
+  }
+  sync();
+  HermitianEigensolver eig(ctx, R);
+  const Precision f64 = Precision::FP64;
+  auto to_host = [&](const double* d, long count) {
+    std::vector<double> h(count);
+    check(hipMemcpyAsync(h.data(), d, 8 * std::size_t(count), hipMemcpyDeviceToHost, s), "to host");
+    sync();
+    return h;
+  };
+  int* info = reinterpret_cast<int*>(small.p);
+  auto potrf_upper = [&](double* a, long m) {
+    check(rocsolver_zpotrf(ctx.blas(), rocblas_fill_upper, rocblas_int(m), reinterpret_cast<rocblas_double_complex*>(a),
+                           rocblas_int(m), info),
+          "rocsolver_zpotrf");
+    int pinfo = 0;
+    check(hipMemcpyAsync(&pinfo, info, sizeof(int), hipMemcpyDeviceToHost, s), "info");
+    sync();
+    return pinfo == 0;
+  };
+  DavidsonOutput out;
+  out.eigenvalues.assign(R, 0.);
+  out.residual_norms.assign(R, 0.);
+  bool converged = false;
+  for (int sweep = 0; sweep < 2 && !converged; ++sweep) {
+    for (long first = 0; first < R; first += G) {
+      const long count = std::min(G, R - first);
+      DavidsonParams lp = params;
+      lp.roots = count;
+      lp.band_group_local = 1;
+      lp.capacity = std::min(n, 5 * G);
+      if (first) {
+        // S-orthonormalise the completed bands (Cholesky), src/band_group_solver.cpp:100-123
+        op.apply(1, x.p, first, nullptr, tmp.p, f64);
+        ctx.zgemm(f64, 1, 0, first, first, n, 1.0, x.p, n, tmp.p, n, ss.p, first, nullptr, 0, 0.0);
+        check(launch_hermitise(ss.p, first, first, hh.p, first, s), "hermitise");
+        if (!potrf_upper(hh.p, first)) throw std::runtime_error("nonpositive completed-band overlap");
+        check(rocsolver_ztrtri(ctx.blas(), rocblas_fill_upper, rocblas_diagonal_non_unit, rocblas_int(first),
+                               reinterpret_cast<rocblas_double_complex*>(hh.p), rocblas_int(first), info),
+              "rocsolver_ztrtri");
+        check(launch_zero_lower(hh.p, first, first, s), "zero lower");
+        ctx.zgemm(f64, 0, 0, n, first, first, 1.0, x.p, n, hh.p, first, tmp.p, n, nullptr, 0, 0.0);
+        check(hipMemcpyAsync(x.p, tmp.p, 16 * std::size_t(n) * first, hipMemcpyDeviceToDevice, s), "locked");
+        lp.excluded = x.p;
+        lp.excluded_count = first;
+      }
+      std::vector<C> seed;
+      if (sweep || initial) {
+        seed.resize(std::size_t(n) * count);
+        check(hipMemcpyAsync(seed.data(), x.p + 2 * first * n, 16 * std::size_t(n) * count, hipMemcpyDeviceToHost,
+                             s),
+              "seed");
+        sync();
+      }
+      const DavidsonOutput r = davidson(op, lp, seed.empty() ? nullptr : seed.data());
+      if (params.verbose)
+        std::fprintf(stderr, "band_group sweep=%d first=%ld count=%ld iterations=%ld converged=%d\n", sweep, first,
+                     count, r.stats.iterations, int(r.stats.converged));
+      out.stats.iterations += r.stats.iterations;
+      out.stats.h_columns_exact += r.stats.h_columns_exact;
+      out.stats.h_columns_search += r.stats.h_columns_search;
+      check(hipMemcpyAsync(x.p + 2 * first * n, r.eigenvectors.data(), 16 * std::size_t(n) * count,
+                           hipMemcpyHostToDevice, s),
+            "group vectors");
+      sync();
+    }
+    // global roots x roots rotation on fresh H/S, src/band_group_solver.cpp:158-172
+    op.apply(2, x.p, R, hx.p, sx.p, f64);
+    ctx.zgemm(f64, 1, 0, R, R, n, 1.0, x.p, n, hx.p, n, rot.p, R, nullptr, 0, 0.0);
+    check(launch_hermitise(rot.p, R, R, hh.p, R, s), "hermitise");
+    ctx.zgemm(f64, 1, 0, R, R, n, 1.0, x.p, n, sx.p, n, rot.p, R, nullptr, 0, 0.0);
+    check(launch_hermitise(rot.p, R, R, ss.p, R, s), "hermitise");
+    // generalized pencil by Cholesky reduction: S = L L^H, C = L^-1 H L^-H, Y, V = L^-H Y
+    const rocblas_double_complex one{1.0, 0.0};
+    auto* L = reinterpret_cast<rocblas_double_complex*>(ss.p);
+    auto* H = reinterpret_cast<rocblas_double_complex*>(hh.p);
+    check(rocsolver_zpotrf(ctx.blas(), rocblas_fill_lower, rocblas_int(R), L, rocblas_int(R), info), "zpotrf");
+    {
+      int pinfo = 0;
+      check(hipMemcpyAsync(&pinfo, info, sizeof(int), hipMemcpyDeviceToHost, s), "info");
+      sync();
+      if (pinfo) throw std::runtime_error("band-group global rotation failed");
+    }
+    check(rocblas_ztrsm(ctx.blas(), rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                        rocblas_diagonal_non_unit, rocblas_int(R), rocblas_int(R), &one, L, rocblas_int(R), H,
+                        rocblas_int(R)),
+          "ztrsm");
+    check(rocblas_ztrsm(ctx.blas(), rocblas_side_right, rocblas_fill_lower, rocblas_operation_conjugate_transpose,
+                        rocblas_diagonal_non_unit, rocblas_int(R), rocblas_int(R), &one, L, rocblas_int(R), H,
+                        rocblas_int(R)),
+          "ztrsm");
+    check(launch_hermitise(hh.p, R, R, tmp.p, R, s), "hermitise");
+    eig.solve(tmp.p, R, R, rot.p);
+    check(rocblas_ztrsm(ctx.blas(), rocblas_side_left, rocblas_fill_lower, rocblas_operation_conjugate_transpose,
+                        rocblas_diagonal_non_unit, rocblas_int(R), rocblas_int(R), &one, L, rocblas_int(R),
+                        reinterpret_cast<rocblas_double_complex*>(rot.p), rocblas_int(R)),
+          "ztrsm");
+    ctx.zgemm(f64, 0, 0, n, R, R, 1.0, x.p, n, rot.p, R, tmp.p, n, nullptr, 0, 0.0);
+    check(hipMemcpyAsync(x.p, tmp.p, 16 * std::size_t(n) * R, hipMemcpyDeviceToDevice, s), "rotate");
+    ctx.zgemm(f64, 0, 0, n, R, R, 1.0, sx.p, n, rot.p, R, tmp.p, n, nullptr, 0, 0.0);
+    // S-normalise, then certify on fresh FP64 H and S (src/band_group_solver.cpp:173-196)
+    double* o = small.p + 8;
+    check(launch_real_dots(x.p, n, tmp.p, n, n, R, o, s), "metric");
+    std::vector<double> metric = to_host(o, R);
+    for (long j = 0; j < R; ++j) {
+      if (!(metric[j] > 0)) throw std::runtime_error("band-group invalid final metric");
+      metric[j] = std::sqrt(metric[j]);
+    }
+    check(hipMemcpyAsync(o, metric.data(), 8 * std::size_t(R), hipMemcpyHostToDevice, s), "norms");
+    sync();
+    check(launch_divide_columns(x.p, n, n, R, o, s), "normalise");
+    op.apply(2, x.p, R, hx.p, sx.p, f64);
+    check(launch_real_dots(x.p, n, hx.p, n, n, R, o, s), "energy");
+    const std::vector<double> num = to_host(o, R);
+    check(launch_real_dots(x.p, n, sx.p, n, n, R, o, s), "metric");
+    const std::vector<double> den = to_host(o, R);
+    for (long j = 0; j < R; ++j) out.eigenvalues[j] = num[j] / den[j];
+    check(hipMemcpyAsync(o, out.eigenvalues.data(), 8 * std::size_t(R), hipMemcpyHostToDevice, s), "values");
+    sync();
+    check(launch_fma_residual(hx.p, sx.p, n, n, R, o, tmp.p, s), "residual");
+    check(launch_col_norms(tmp.p, n, n, R, o, s), "norms");
+    out.residual_norms = to_host(o, R);
+    converged = true;
+    for (double r : out.residual_norms) converged = converged && r <= params.tolerance;
+  }
+  out.eigenvectors.resize(std::size_t(n) * R);
+  check(hipMemcpyAsync(out.eigenvectors.data(), x.p, 16 * std::size_t(n) * R, hipMemcpyDeviceToHost, s), "vectors");
+  sync();
+  out.stats.converged = converged;
+  out.stats.seconds_total = now() - t_start;
+  return out;
 }
 
 }  // namespace sdft_hip

@@ -24,6 +24,14 @@ struct DavidsonParams {
                                   // FP64 images for those, instead of re-applying FP64 to the whole subspace
   int verbose = 0;
   int timing = 0;                 // synchronise phases and accumulate per-phase times
+  long band_group_size = 0;       // > 0: bounded band-group solver (src/band_group_solver.cpp:57-211)
+  // Local solve of one band group (src/band_group_solver.cpp:84-100): capacity
+  // override, completed bands to keep out of the search space (S-orthonormal,
+  // device n x excluded_count), and no strict polish (src/pw_core.cpp:1273).
+  long capacity = 0;
+  const double* excluded = nullptr;
+  long excluded_count = 0;
+  int band_group_local = 0;
 };
 
 struct DavidsonStats {
@@ -32,6 +40,8 @@ struct DavidsonStats {
   long h_columns_search = 0, h_columns_exact = 0;
   long handoff_iteration = -1;
   double seconds_total = 0, seconds_apply = 0, seconds_gemm = 0, seconds_ritz = 0, seconds_ortho = 0;
+  long cholesky_blocks = 0;  // correction blocks normalised by Cholesky QR
+  long rr_cholesky = 0;      // Rayleigh-Ritz solves that needed the Cholesky reduction
   std::vector<double> max_residual_history;
 };
 
@@ -43,5 +53,10 @@ struct DavidsonOutput {
 
 DavidsonOutput davidson(DeviceOperator& op, const DavidsonParams& params,
                         const std::complex<double>* initial /* n x roots or nullptr */);
+// The bounded band-group solver: local Davidson solves of band_group_size
+// roots with the completed bands excluded, then a global roots x roots
+// Rayleigh-Ritz rotation certified on fresh FP64 H and S (two sweeps).
+DavidsonOutput davidson_band_groups(DeviceOperator& op, const DavidsonParams& params,
+                                    const std::complex<double>* initial);
 
 }  // namespace sdft_hip

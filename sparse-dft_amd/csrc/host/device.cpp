@@ -12,6 +12,7 @@
 #include <tuple>
 
 #include "../hip/fft_lengths.hpp"
+#include "snapshot.hpp"
 
 namespace sdft_hip {
 
@@ -79,6 +80,28 @@ HostGridPlan build_grid_plan(std::array<int, 3> shape, const std::uint64_t* indi
     if (p.stick_g[std::size_t(sid) * n2 + z] >= 0) throw std::invalid_argument("HIP: duplicate plane-wave index");
     p.stick_g[std::size_t(sid) * n2 + z] = int(g);
   }
+  // x-lines holding sticks: the signed Miller range [lo, hi] of x, wrapped into
+  // at most two ranges of natural x (the whole axis when they would overlap)
+  const int n0 = shape[0], n1 = shape[1];
+  int lo = n0, hi = -n0;
+  for (int key : xy) {
+    const int x = key / n1, h = x <= n0 / 2 ? x : x - n0;
+    lo = std::min(lo, h);
+    hi = std::max(hi, h);
+  }
+  if (xy.empty() || hi - lo + 1 >= n0) {
+    p.xstart[0] = 0;
+    p.xlines[0] = n0;
+  } else {
+    if (hi >= 0) {
+      p.xstart[0] = std::max(lo, 0);
+      p.xlines[0] = hi + 1 - p.xstart[0];
+    }
+    if (lo < 0) {
+      p.xstart[1] = n0 + lo;
+      p.xlines[1] = std::min(hi, -1) - lo + 1;
+    }
+  }
   return p;
 }
 
@@ -92,6 +115,7 @@ HostGridPlan build_full_plan(std::array<int, 3> shape) {
   p.stick_g.resize(std::size_t(p.count));
   for (int s = 0; s < planes; ++s) p.stick_xy[s] = s;
   for (long i = 0; i < p.count; ++i) p.stick_g[i] = int(i);
+  p.xlines[0] = shape[0];
   return p;
 }
 
@@ -106,6 +130,10 @@ void DeviceGrid::upload(const HostGridPlan& h, hipStream_t s) {
   dev.nsticks = h.nsticks();
   dev.stick_g = stick_g.p;
   dev.stick_xy = stick_xy.p;
+  for (int r = 0; r < 2; ++r) {
+    dev.xstart[r] = h.xstart[r];
+    dev.xlines[r] = h.xlines[r];
+  }
   for (int a = 0; a < 3; ++a) {
     const int L = h.shape[a];
     std::vector<double> t64(2 * L);
@@ -143,6 +171,7 @@ Context::Context(int device) : device_(device) {
 
 Context::~Context() {
   (void)hipStreamSynchronize(stream_);
+  for (hipEvent_t e : events_) (void)hipEventDestroy(e);
   if (blas_) rocblas_destroy_handle(blas_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -153,6 +182,44 @@ void Context::record(const std::string& name, double seconds, std::size_t calls)
   c.calls += calls;
 }
 
+void Context::clock_enable(bool on) {
+  sync();
+  clock_on_ = on;
+  pending_.clear();
+  events_used_ = 0;
+  for (auto& c : clock_) c = ClockStat{};
+}
+
+LaunchEvents Context::clock_launch(int kind, double work) {
+  if (!clock_on_) return {};
+  if (kind < 0 || kind >= kClockKinds) throw std::invalid_argument("HIP: invalid kernel clock kind");
+  while (events_.size() < events_used_ + 2) {
+    hipEvent_t e{};
+    check(hipEventCreate(&e), "hipEventCreate");
+    events_.push_back(e);
+  }
+  pending_.push_back({kind, work, events_used_});
+  LaunchEvents ev{events_[events_used_], events_[events_used_ + 1]};
+  events_used_ += 2;
+  return ev;
+}
+
+Context::ClockStat Context::clock_stat(int kind) {
+  if (kind < 0 || kind >= kClockKinds) throw std::invalid_argument("HIP: invalid kernel clock kind");
+  sync();
+  for (const auto& p : pending_) {
+    float ms = 0.f;
+    check(hipEventElapsedTime(&ms, events_[p.first_event], events_[p.first_event + 1]), "hipEventElapsedTime");
+    auto& c = clock_[p.kind];
+    c.launches += 1;
+    c.milliseconds += ms;
+    c.work += p.work;
+  }
+  pending_.clear();
+  events_used_ = 0;
+  return clock_[kind];
+}
+
 void Context::zgemm(Precision prec, int aop, int bop, long M, long N, long K, double alpha, const double* A, long lda,
                     const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta) {
   if (M <= 0 || N <= 0) return;
@@ -160,9 +227,20 @@ void Context::zgemm(Precision prec, int aop, int bop, long M, long N, long K, do
     throw std::overflow_error("HIP: GEMM dimension overflow");
   const std::size_t bytes = zgemm_workspace_bytes(int(M), int(N), int(K));
   double* w = bytes ? gemm_work(bytes) : nullptr;
+  const LaunchEvents ev = K > 0 ? clock_launch(prec == Precision::FP64 ? kGemmFP64 : kGemmFP32, 8.0 * M * N * K)
+                                : LaunchEvents{};
   check(sdft_hip::zgemm(prec, aop, bop, int(M), int(N), int(K), alpha, A, lda, B, ldb, C, ldc, D, ldd, beta, w,
-                        bytes ? gemm_work_bytes() : 0, stream_),
+                        bytes ? gemm_work_bytes() : 0, stream_, &ev),
         "zgemm launch");
+}
+
+// Algorithmic bytes of one local-Hamiltonian launch over m columns:
+// m (2 npw c + 4 N c) + N r + npw (4 + r), c/r = complex/real element bytes of
+// the arithmetic (each 3-D transform reads and writes the grid once; V(r), the
+// sphere indices and T are read once per launch).
+double local_algorithmic_bytes(long npw, long count, long m, bool fp64) {
+  const double c = fp64 ? 16.0 : 8.0, r = fp64 ? 8.0 : 4.0;
+  return double(m) * (2.0 * npw * c + 4.0 * count * c) + count * r + npw * (4.0 + r);
 }
 
 // ------------------------------------------------------------ operator
@@ -325,6 +403,7 @@ DeviceOperator::DeviceOperator(Context& ctx, const OperatorDesc& d) : ctx_(ctx) 
     qblk_.upload(reinterpret_cast<const double*>(qbl.data()), 2 * qbl.size(), s);
     atom_offset_.upload(offs.data(), offs.size(), s);
     atom_rows_.upload(rws.data(), rws.size(), s);
+    atom_rows_host_ = rws;
     block_offset_.upload(boffs.data(), boffs.size(), s);
     ctx_.sync();
   }
@@ -332,6 +411,122 @@ DeviceOperator::DeviceOperator(Context& ctx, const OperatorDesc& d) : ctx_(ctx) 
   const std::size_t per_col = std::size_t(grid_.dev.nsticks) * shape_[2] * 16;
   chunk_ = std::max<std::size_t>(1, std::min<std::size_t>(512, (std::size_t(512) << 20) / std::max<std::size_t>(1, per_col)));
   ctx_.sync();
+}
+
+DeviceOperator::DeviceOperator(Context& ctx, const SnapshotArrays& sa) : ctx_(ctx) {
+  n_ = std::size_t(sa.n);
+  np_ = std::size_t(sa.projectors);
+  count_ = long(sa.count);
+  reg_ = sa.regularization;
+  form_ = int(sa.form);
+  for (int a = 0; a < 3; ++a) {
+    shape_[a] = int(sa.shape[a]);
+    int l1 = 0, l2 = 0;
+    if (!length_split(shape_[a], l1, l2)) throw std::invalid_argument("HIP: snapshot FFT length has no device transform");
+  }
+  if (sa.indices.size() != n_ || sa.kinetic.size() != n_ || sa.h_diagonal.size() != n_ ||
+      sa.s_diagonal.size() != n_ || sa.local_real.size() != sa.count || sa.all_b.size() != np_ * n_)
+    throw std::invalid_argument("HIP: inconsistent snapshot arrays");
+  hipStream_t s = ctx_.stream();
+  indices_ = sa.indices;
+  kinetic_ = sa.kinetic;
+  h_diag_ = sa.h_diagonal;
+  s_diag_ = sa.s_diagonal;
+  const HostGridPlan plan = build_grid_plan(shape_, indices_.data(), n_);
+  grid_.upload(plan, s);
+  {
+    DevBuf<double> nat;
+    nat.upload(sa.local_real.data(), sa.local_real.size(), s);
+    v64_.reserve(std::size_t(count_));
+    v32_.reserve(std::size_t(count_));
+    check(launch_potential_to_slots(grid_.dev, nat.p, v64_.p, v32_.p, s), "potential to slots");
+    ctx_.sync();
+  }
+  grid_.dev.v64 = v64_.p;
+  grid_.dev.v32 = v32_.p;
+  kinetic_d_.upload(kinetic_.data(), n_, s);
+  if (np_) {
+    const double* b = reinterpret_cast<const double*>(sa.all_b.data());
+    b64_.upload(b, 2 * sa.all_b.size(), s);
+    std::vector<float> b32(2 * sa.all_b.size());
+    for (std::size_t i = 0; i < b32.size(); ++i) b32[i] = float(b[i]);
+    b32_.upload(b32.data(), b32.size(), s);
+    if (form_ == SnapshotArrays::Blocks) {
+      std::vector<int> offs;
+      std::vector<long> boffs;
+      std::size_t off = 0, sq = 0;
+      for (std::uint64_t c : sa.channels) {
+        offs.push_back(int(off));
+        atom_rows_host_.push_back(int(c));
+        boffs.push_back(long(sq));
+        off += c;
+        sq += c * c;
+      }
+      if (off != np_ || sa.onsite_d.size() != sq || sa.onsite_q.size() != sq)
+        throw std::invalid_argument("HIP: inconsistent snapshot atom blocks");
+      atoms_ = int(sa.channels.size());
+      dblk_.upload(reinterpret_cast<const double*>(sa.onsite_d.data()), 2 * sq, s);
+      qblk_.upload(reinterpret_cast<const double*>(sa.onsite_q.data()), 2 * sq, s);
+      atom_offset_.upload(offs.data(), offs.size(), s);
+      atom_rows_.upload(atom_rows_host_.data(), atom_rows_host_.size(), s);
+      block_offset_.upload(boffs.data(), boffs.size(), s);
+    } else {
+      const std::size_t cols = form_ == SnapshotArrays::Dense ? np_ : n_;
+      if (sa.onsite_d.size() != np_ * cols || sa.onsite_q.size() != np_ * cols)
+        throw std::invalid_argument("HIP: inconsistent snapshot onsite matrices");
+      onsite_d_.upload(reinterpret_cast<const double*>(sa.onsite_d.data()), 2 * sa.onsite_d.size(), s);
+      onsite_q_.upload(reinterpret_cast<const double*>(sa.onsite_q.data()), 2 * sa.onsite_q.size(), s);
+    }
+  }
+  const std::size_t per_col = std::size_t(grid_.dev.nsticks) * shape_[2] * 16;
+  chunk_ = std::max<std::size_t>(1, std::min<std::size_t>(512, (std::size_t(512) << 20) / std::max<std::size_t>(1, per_col)));
+  ctx_.sync();
+}
+
+SnapshotArrays DeviceOperator::snapshot(std::size_t roots, std::size_t iterations, double tolerance,
+                                        const std::complex<double>* initial) {
+  using C = std::complex<double>;
+  SnapshotArrays sa;
+  sa.form = SnapshotArrays::Form(form_);
+  sa.n = n_;
+  sa.count = std::uint64_t(count_);
+  for (int a = 0; a < 3; ++a) sa.shape[a] = std::uint64_t(shape_[a]);
+  sa.projectors = np_;
+  sa.roots = roots;
+  sa.iterations = iterations;
+  sa.regularization = reg_;
+  sa.tolerance = tolerance;
+  sa.indices = indices_;
+  sa.kinetic = kinetic_;
+  sa.h_diagonal = h_diag_;
+  sa.s_diagonal = s_diag_;
+  sa.local_real = local_real_natural();
+  hipStream_t s = ctx_.stream();
+  auto down = [&](std::vector<C>& dst, const double* src, std::size_t count) {
+    dst.resize(count);
+    if (count) check(hipMemcpyAsync(dst.data(), src, 16 * count, hipMemcpyDeviceToHost, s), "snapshot copy");
+  };
+  if (np_) {
+    down(sa.all_b, b64_.p, np_ * n_);
+    if (form_ == SnapshotArrays::Blocks) {
+      std::size_t sq = 0;
+      for (int r : atom_rows_host_) {
+        sa.channels.push_back(std::uint64_t(r));
+        sq += std::size_t(r) * r;
+      }
+      down(sa.onsite_d, dblk_.p, sq);
+      down(sa.onsite_q, qblk_.p, sq);
+    } else {
+      const std::size_t cols = form_ == SnapshotArrays::Dense ? np_ : n_;
+      down(sa.onsite_d, onsite_d_.p, np_ * cols);
+      down(sa.onsite_q, onsite_q_.p, np_ * cols);
+    }
+  } else {
+    sa.form = SnapshotArrays::Dense;  // an operator without projectors is stored as PWO2
+  }
+  ctx_.sync();
+  if (initial) sa.initial.assign(initial, initial + n_ * roots);
+  return sa;
 }
 
 std::vector<double> DeviceOperator::local_real_natural() {
@@ -362,6 +557,110 @@ void DeviceOperator::apply(int which, const double* x, long columns, double* hy,
   }
 }
 
+void DeviceOperator::project(const double* x, long columns, double* p, Precision prec) {
+  if (columns <= 0 || !x || !p || x == p || !np_) throw std::invalid_argument("HIP: invalid PAW projection buffers");
+  ctx_.zgemm(prec, 0, 0, long(np_), columns, long(n_), 1.0, b64_.p, long(np_), x, long(n_), p, long(np_), nullptr, 0,
+             0.0);
+}
+
+// src/density_functionals.cpp:406-460. density(r) = sum_b occ_b / volume
+// |psi_b(r)|^2 with psi_b the unnormalised backward transform of the band's
+// coefficients; result(G) = forward transform / N at the support points. The
+// grid is alias free for every G - G' and every support point (any such grid
+// gives the same coefficients; the reference notes the same for its
+// SDFT_FAST_DENSITY_GRID enlargement, src/density_functionals.cpp:356-366).
+std::vector<std::complex<double>> DeviceOperator::density(const double* x, long bands, const double* occupations,
+                                                          double volume, const int* support, long nsupport) {
+  if (bands <= 0 || !x || !occupations || !support || nsupport <= 0 || !std::isfinite(volume) || volume <= 0)
+    throw std::invalid_argument("invalid plane-wave FFT density dimensions");
+  for (long b = 0; b < bands; ++b)
+    if (!std::isfinite(occupations[b]) || occupations[b] < 0 || occupations[b] > 2)
+      throw std::invalid_argument("invalid FFT spin-summed occupation");
+  // Miller indices of the basis on the operator grid
+  std::vector<int> hkl(3 * n_);
+  int lo[3]{0, 0, 0}, hi[3]{0, 0, 0}, m[3]{0, 0, 0};
+  for (std::size_t g = 0; g < n_; ++g) {
+    const std::uint64_t i = indices_[g];
+    const int c[3]{int(i / (std::uint64_t(shape_[1]) * shape_[2])), int((i / shape_[2]) % shape_[1]),
+                   int(i % shape_[2])};
+    for (int a = 0; a < 3; ++a) {
+      const int h = c[a] <= shape_[a] / 2 ? c[a] : c[a] - shape_[a];
+      hkl[3 * g + a] = h;
+      lo[a] = std::min(lo[a], h);
+      hi[a] = std::max(hi[a], h);
+    }
+  }
+  for (long j = 0; j < nsupport; ++j)
+    for (int a = 0; a < 3; ++a) m[a] = std::max(m[a], std::abs(support[3 * j + a]));
+  std::array<int, 3> shape{};
+  for (int a = 0; a < 3; ++a) {
+    m[a] = std::max(m[a], hi[a] - lo[a]);
+    int len = std::max(3, 2 * m[a] + 1), l1 = 0, l2 = 0;
+    while (len <= 128 && !length_split(len, l1, l2)) ++len;
+    if (len > 128) throw std::invalid_argument("HIP: density grid beyond the device FFT lengths");
+    shape[a] = len;
+  }
+  if (std::size_t(shape[0]) * shape[1] * 16 > std::size_t(max_lds_bytes()))
+    throw std::invalid_argument("HIP: density grid plane exceeds the LDS of one workgroup");
+  const long count = long(shape[0]) * shape[1] * shape[2];
+  std::vector<std::uint64_t> idx(n_);
+  for (std::size_t g = 0; g < n_; ++g) idx[g] = grid_index(shape, hkl[3 * g], hkl[3 * g + 1], hkl[3 * g + 2]);
+  hipStream_t s = ctx_.stream();
+  DeviceGrid gd, gf;
+  gd.upload(build_grid_plan(shape, idx.data(), n_), s);
+  gf.upload(build_full_plan(shape), s);
+  const long chunk = std::max<long>(1, std::min<long>(bands, (256L << 20) / (count * 16)));
+  DevBuf<char> t1, t2;
+  t1.reserve(std::size_t(std::max<long>(chunk * gd.dev.nsticks * shape[2], count)) * 16);
+  t2.reserve(std::size_t(chunk * count) * 16);
+  DevBuf<double> rho, nat, weights, out;
+  rho.reserve(std::size_t(count));
+  nat.reserve(std::size_t(2 * count));
+  weights.reserve(std::size_t(chunk));
+  out.reserve(std::size_t(2 * nsupport));
+  check(hipMemsetAsync(rho.p, 0, sizeof(double) * std::size_t(count), s), "zero density");
+  std::vector<double> w(chunk);
+  for (long b0 = 0; b0 < bands; b0 += chunk) {
+    const long c = std::min(chunk, bands - b0);
+    for (long j = 0; j < c; ++j) w[j] = occupations[b0 + j] / volume;
+    check(hipMemcpyAsync(weights.p, w.data(), sizeof(double) * std::size_t(c), hipMemcpyHostToDevice, s), "weights");
+    check(launch_backward_planes(gd.dev, x + 2 * b0 * long(n_), long(n_), c, t1.p, t2.p, s), "band transforms");
+    check(launch_density_accumulate(t2.p, count, int(c), weights.p, rho.p, s), "density accumulation");
+    ctx_.sync();  // the host weights are reused by the next chunk
+  }
+  check(launch_slots_to_natural_complex(gf.dev, rho.p, nat.p, s), "density to natural order");
+  check(launch_backward_planes(gf.dev, nat.p, 0, 1, t1.p, t2.p, s), "density transform");
+  DevBuf<int> keys;
+  keys.upload(support, std::size_t(3 * nsupport), s);
+  check(launch_gather_support(gf.dev, t2.p, keys.p, nsupport, out.p, s), "support gather");
+  std::vector<std::complex<double>> result(nsupport);
+  check(hipMemcpyAsync(result.data(), out.p, 16 * std::size_t(nsupport), hipMemcpyDeviceToHost, s), "density copy");
+  ctx_.sync();
+  return result;
+}
+
+// Per-atom Q_a P_a in the compensated FP64 order (atom blocks) or the dense
+// Q P (PWO2); the folded form keeps no Q of its own.
+void DeviceOperator::projector_metric(const double* p, long columns, double* qp) {
+  if (columns <= 0 || !p || !qp || p == qp) throw std::invalid_argument("HIP: invalid projector metric buffers");
+  if (!np_) return;
+  const long np = long(np_);
+  if (form_ == 3) {
+    // the onsite kernel writes Q P into the second column block of its output
+    w_.reserve(std::size_t(2 * np * 2 * columns));
+    check(launch_atom_blocks(Precision::FP64, int(np), int(columns), atoms_, atom_offset_.p, atom_rows_.p,
+                             block_offset_.p, dblk_.p, qblk_.p, p, np, w_.p, np, 2, ctx_.stream()),
+          "projector metric");
+    check(hipMemcpyAsync(qp, w_.p + 2 * np * columns, 16 * std::size_t(np) * columns, hipMemcpyDeviceToDevice,
+                         ctx_.stream()),
+          "projector metric copy");
+  } else if (form_ == 2) {
+    ctx_.zgemm(Precision::FP64, 0, 0, np, columns, np, 1.0, onsite_q_.p, np, p, np, qp, np, nullptr, 0, 0.0);
+  } else {
+    throw std::invalid_argument("HIP: a folded (PWO1) operator has no projector metric");
+  }
+}
+
 // src/plane_wave_operator.cpp:218-374: local part through the stick pipeline,
 // P = B X once for the pair, onsite D/Q blocks, then B^H (.) added with the
 // kinetic/local H or the (1+reg) identity of S folded into the GEMM epilogue.
@@ -369,7 +668,21 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
   hipStream_t s = ctx_.stream();
   const long n = long(n_), np = long(np_);
   const bool want_h = which != 1, want_s = which != 0;
-  if (np) {
+  if (np && form_ != 3) {
+    // PWO2 dense D/Q on P = B X, or PWO1 folded DB/QB on X
+    // (src/plane_wave_operator.cpp:290-301)
+    w_.reserve(std::size_t(2 * np * 2 * c));
+    const bool dense = form_ == 2;
+    if (dense) {
+      p_.reserve(std::size_t(2 * np * c));
+      ctx_.zgemm(prec, 0, 0, np, c, n, 1.0, b64_.p, np, x, n, p_.p, np, nullptr, 0, 0.0);
+    }
+    const double* rhs = dense ? p_.p : x;
+    const long inner = dense ? np : n, ldr = dense ? np : n;
+    if (want_h) ctx_.zgemm(prec, 0, 0, np, c, inner, 1.0, onsite_d_.p, np, rhs, ldr, w_.p, np, nullptr, 0, 0.0);
+    if (want_s)
+      ctx_.zgemm(prec, 0, 0, np, c, inner, 1.0, onsite_q_.p, np, rhs, ldr, w_.p + 2 * np * c, np, nullptr, 0, 0.0);
+  } else if (np) {
     p_.reserve(std::size_t(2 * np * c));
     w_.reserve(std::size_t(2 * np * 2 * c));
     if (prec == Precision::FP64)
@@ -377,8 +690,9 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
     else {
       const std::size_t bytes = zgemm_workspace_bytes(int(np), int(c), int(n));
       double* wk = bytes ? ctx_.gemm_work(bytes) : nullptr;
+      const LaunchEvents ev = ctx_.clock_launch(Context::kGemmFP32, 8.0 * np * c * n);
       check(zgemm_f32_storage(0, 0, int(np), int(c), int(n), 1.0, b32_.p, np, x, n, p_.p, np, nullptr, 0, 0.0, wk,
-                              bytes ? ctx_.gemm_work_bytes() : 0, s),
+                              bytes ? ctx_.gemm_work_bytes() : 0, s, &ev),
             "projection");
     }
     const int mask = (want_h ? 1 : 0) | (want_s ? 2 : 0);
@@ -390,10 +704,15 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
     const std::size_t bytes = std::size_t(c) * grid_.dev.nsticks * shape_[2] * (prec == Precision::FP64 ? 16 : 8);
     t1_.reserve(bytes);
     t2_.reserve(bytes);
-    if (prec == Precision::FP64)
+    const bool f64 = prec == Precision::FP64;
+    const LaunchEvents ev = ctx_.clock_launch(f64 ? Context::kLocalFP64 : Context::kLocalFP32,
+                                              local_algorithmic_bytes(n, count_, c, f64));
+    if (ev.begin) check(hipEventRecord(ev.begin, s), "hipEventRecord");
+    if (f64)
       check(launch_local_h<double>(grid_.dev, x, n, c, kinetic_d_.p, hy, n, t1_.p, t2_.p, s), "local H");
     else
       check(launch_local_h<float>(grid_.dev, x, n, c, kinetic_d_.p, hy, n, t1_.p, t2_.p, s), "local H");
+    if (ev.end) check(hipEventRecord(ev.end, s), "hipEventRecord");
   }
   auto backproject = [&](const double* w, double* out, const double* d, double beta) {
     if (prec == Precision::FP64)
@@ -401,8 +720,9 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
     else {
       const std::size_t bytes = zgemm_workspace_bytes(int(n), int(c), int(np));
       double* wk = bytes ? ctx_.gemm_work(bytes) : nullptr;
+      const LaunchEvents ev = ctx_.clock_launch(Context::kGemmFP32, 8.0 * n * c * np);
       check(zgemm_f32_storage(1, 0, int(n), int(c), int(np), 1.0, b32_.p, np, w, np, out, n, d, n, beta, wk,
-                              bytes ? ctx_.gemm_work_bytes() : 0, s),
+                              bytes ? ctx_.gemm_work_bytes() : 0, s, &ev),
             "back projection");
     }
   };

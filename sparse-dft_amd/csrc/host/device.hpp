@@ -6,6 +6,7 @@
 #include <rocblas/rocblas.h>
 
 #include <array>
+#include <complex>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -61,6 +62,7 @@ struct HostGridPlan {
   std::vector<int> stick_xy;  // nsticks
   std::vector<int> stick_g;   // nsticks * n2
   long count = 0;
+  int xstart[2]{}, xlines[2]{};  // x-lines holding sticks (two ranges of natural x)
   int nsticks() const { return int(stick_xy.size()); }
 };
 HostGridPlan build_grid_plan(std::array<int, 3> shape, const std::uint64_t* indices, std::size_t n);
@@ -92,6 +94,18 @@ class Context {
   void record(const std::string& name, double seconds, std::size_t calls = 1);
   std::map<std::string, PerfCounter> counters;
 
+  // Kernel clock: event pairs on this context's stream around every MFMA GEMM
+  // kernel and every local-Hamiltonian pipeline launch, summed per kind.
+  enum ClockKind { kGemmFP64 = 0, kGemmFP32 = 1, kLocalFP64 = 2, kLocalFP32 = 3, kClockKinds = 4 };
+  struct ClockStat {
+    long launches = 0;
+    double milliseconds = 0, work = 0;  // work: flops (GEMM) or algorithmic bytes (local H)
+  };
+  void clock_enable(bool on);  // clears the sums either way
+  // Events to record for one launch (both null when the clock is off).
+  LaunchEvents clock_launch(int kind, double work);
+  ClockStat clock_stat(int kind);  // synchronises the stream and folds pending pairs
+
   // caches for the drop-in host-pointer entry points (identity tokens)
   struct Resident {
     std::uint64_t identity = 0;
@@ -121,7 +135,20 @@ class Context {
   rocblas_handle blas_{};
   std::string name_;
   DevBuf<double> work_;
+  bool clock_on_ = false;
+  std::vector<hipEvent_t> events_;
+  std::size_t events_used_ = 0;
+  struct PendingLaunch {
+    int kind;
+    double work;
+    std::size_t first_event;
+  };
+  std::vector<PendingLaunch> pending_;
+  ClockStat clock_[kClockKinds];
 };
+
+double local_algorithmic_bytes(long npw, long count, long m, bool fp64);
+struct SnapshotArrays;
 
 struct OperatorDesc {
   std::size_t n = 0;
@@ -147,8 +174,25 @@ struct OperatorDesc {
 class DeviceOperator {
  public:
   DeviceOperator(Context& ctx, const OperatorDesc& desc);
+  // From the arrays of a solver snapshot (src/plane_wave_operator.cpp:587-653):
+  // V(r), indices, kinetic and diagonals as stored; the nonlocal part in the
+  // snapshot's form (PWO3 atom blocks, PWO2 dense D/Q, PWO1 folded DB/QB).
+  DeviceOperator(Context& ctx, const SnapshotArrays& snapshot);
+  // The operator's arrays in its snapshot form, with a Davidson request.
+  SnapshotArrays snapshot(std::size_t roots, std::size_t iterations, double tolerance,
+                          const std::complex<double>* initial) ;
   // which: 0 = H, 1 = S, 2 = joint pair. x, hy, sy: device pointers, ld = n.
   void apply(int which, const double* x, long columns, double* hy, double* sy, Precision prec);
+  // P = B X (np x columns, ld np) and QP = Q P with the onsite metric
+  // (src/plane_wave_operator.cpp:375-389); device pointers.
+  void project(const double* x, long columns, double* p, Precision prec);
+  // Smooth density Fourier coefficients on `support` (Miller triples) from the
+  // band columns x (device, n x bands) with spin-summed occupations (host),
+  // restating plane_wave_density_fft (src/density_functionals.cpp:406-460) on
+  // the smallest supported alias-free grid.
+  std::vector<std::complex<double>> density(const double* x, long bands, const double* occupations, double volume,
+                                            const int* support, long nsupport);
+  void projector_metric(const double* p, long columns, double* qp);
   std::size_t n() const { return n_; }
   std::size_t np() const { return np_; }
   long count() const { return count_; }
@@ -176,6 +220,9 @@ class DeviceOperator {
   double reg_ = 0;
   std::vector<std::uint64_t> indices_;
   std::vector<double> kinetic_, h_diag_, s_diag_;
+  int form_ = 3;  // nonlocal form: 3 atom blocks, 2 dense D/Q, 1 folded DB/QB
+  std::vector<int> atom_rows_host_;
+  DevBuf<double> onsite_d_, onsite_q_;  // dense np x np or folded np x n (forms 2, 1)
   DeviceGrid grid_;
   DevBuf<double> v64_, kinetic_d_, b64_, dblk_, qblk_;
   DevBuf<float> v32_, b32_;

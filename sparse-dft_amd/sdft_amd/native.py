@@ -33,14 +33,22 @@ class DavidsonParams(ct.Structure):
     _fields_ = [("roots", ct.c_long), ("max_iterations", ct.c_long), ("tolerance", ct.c_double),
                 ("subspace_factor", ct.c_long), ("search_precision", ct.c_int), ("search_handoff", ct.c_double),
                 ("stagnation_window", ct.c_int), ("handoff_restart", ct.c_int), ("verbose", ct.c_int),
-                ("timing", ct.c_int)]
+                ("timing", ct.c_int), ("band_group_size", ct.c_long)]
 
 
 class DavidsonResult(ct.Structure):
     _fields_ = [("eigenvalues", _dp), ("eigenvectors", _dp), ("residual_norms", _dp), ("iterations", ct.c_long),
                 ("converged", ct.c_int), ("h_columns_search", ct.c_long), ("h_columns_exact", ct.c_long),
                 ("handoff_iteration", ct.c_long), ("seconds_total", ct.c_double), ("seconds_apply", ct.c_double),
-                ("seconds_gemm", ct.c_double), ("seconds_ritz", ct.c_double), ("seconds_ortho", ct.c_double)]
+                ("seconds_gemm", ct.c_double), ("seconds_ritz", ct.c_double), ("seconds_ortho", ct.c_double),
+                ("cholesky_blocks", ct.c_long), ("rr_cholesky", ct.c_long)]
+
+
+class SnapshotInfo(ct.Structure):
+    _fields_ = [("format", ct.c_int), ("n", ct.c_size_t), ("count", ct.c_size_t), ("shape", ct.c_size_t * 3),
+                ("projectors", ct.c_size_t), ("atoms", ct.c_size_t), ("roots", ct.c_size_t),
+                ("iterations", ct.c_size_t), ("initial_columns", ct.c_size_t), ("regularization", ct.c_double),
+                ("tolerance", ct.c_double)]
 
 
 class HipError(RuntimeError):
@@ -105,6 +113,13 @@ def grid_indices(shape, hkl):
     return out
 
 
+def snapshot_info(path):
+    """Header and validation of a SDFTPWO1/2/3 snapshot (reads the whole file; no GPU)."""
+    info = SnapshotInfo()
+    _check(lib().sdft_hip_snapshot_info_read(os.fsencode(path), ct.byref(info)))
+    return {f: (tuple(getattr(info, f)) if f == "shape" else getattr(info, f)) for f, _ in SnapshotInfo._fields_}
+
+
 class Context:
     """One HIP stream + rocBLAS/rocSOLVER handle, owned by one host thread."""
 
@@ -130,6 +145,37 @@ class Context:
 
     def synchronize(self):
         _check(lib().sdft_hip_synchronize(self.handle))
+
+    def hermitian_eigen(self, a, k=None):
+        """All eigenvalues (ascending) of the Hermitian matrix a and the lowest k
+        eigenvectors, through the device Rayleigh-Ritz eigensolver; returns
+        (values, vectors, fused)."""
+        a = np.asfortranarray(a, dtype=np.complex128)
+        n = a.shape[0]
+        k = n if k is None else int(k)
+        w = np.zeros(n)
+        v = np.zeros((n, k), dtype=np.complex128, order="F")
+        fused = ct.c_int(0)
+        _check(lib().sdft_hip_hermitian_eigen(self.handle, ct.c_size_t(n), _p(a), ct.c_size_t(k), _p(w),
+                                              _p(v) if k else None, ct.byref(fused)))
+        return w, v, bool(fused.value)
+
+    CLOCK_KINDS = {"gemm_fp64": 0, "gemm_fp32": 1, "local_fp64": 2, "local_fp32": 3}
+
+    def kernel_clock(self, enable):
+        """Start (True) or stop (False) the per-launch HIP-event clock; both clear the sums."""
+        _check(lib().sdft_hip_kernel_clock(self.handle, ct.c_int(int(bool(enable)))))
+
+    def kernel_clock_stats(self):
+        """{kind: {launches, ms, work}}: event-timed sums since the clock was started
+        (work in flops for GEMM kinds, algorithmic bytes for local-H kinds)."""
+        out = {}
+        for name, kind in self.CLOCK_KINDS.items():
+            n, ms, work = ct.c_long(), ct.c_double(), ct.c_double()
+            _check(lib().sdft_hip_kernel_clock_stats(self.handle, ct.c_int(kind), ct.byref(n), ct.byref(ms),
+                                                     ct.byref(work)))
+            out[name] = {"launches": n.value, "ms": ms.value, "work": work.value}
+        return out
 
     # ---- Accelerator mirror (host pointers)
     def zgemm(self, aop, bop, a, b, c=None, add=False, left_identity=0, precision=0, shape=None):
@@ -206,8 +252,13 @@ class Context:
 class Operator:
     """Device-resident plane-wave operator built from a synthetic Problem."""
 
-    def __init__(self, ctx: Context, prob, shape=None):
+    def __init__(self, ctx: Context, prob, shape=None, _handle=None):
         self.ctx = ctx
+        if _handle is not None:
+            self.handle = _handle
+            self.n = None
+            self._describe()
+            return
         self.n = prob.n
         self._keep = dict(
             hkl=np.ascontiguousarray(prob.hkl, dtype=np.int32),
@@ -240,6 +291,53 @@ class Operator:
         self.handle = ct.c_void_p()
         _check(lib().sdft_hip_operator_create(ctx.handle, ct.byref(desc), ct.byref(self.handle)))
         self._keep = None
+        self._describe()
+
+    @classmethod
+    def from_snapshot(cls, ctx: Context, path):
+        """(operator, warm block or None) from a SDFTPWO1/2/3 snapshot."""
+        info = snapshot_info(path)
+        initial = None
+        if info["initial_columns"]:
+            initial = np.zeros((info["n"], info["initial_columns"]), dtype=np.complex128, order="F")
+        h = ct.c_void_p()
+        _check(lib().sdft_hip_snapshot_load(ctx.handle, os.fsencode(path), _p(initial), ct.byref(h)))
+        op = cls(ctx, None, _handle=h)
+        op.n = info["n"]
+        return op, initial
+
+    def project(self, x, precision=0):
+        """P = B X (projectors x m), src/plane_wave_operator.cpp:375-380."""
+        x = np.asfortranarray(x, dtype=np.complex128)
+        p = np.zeros((self.np, x.shape[1]), dtype=np.complex128, order="F")
+        _check(lib().sdft_hip_operator_project(self.handle, _p(x), ct.c_size_t(x.shape[1]), _p(p),
+                                               ct.c_int(precision)))
+        return p
+
+    def projector_metric(self, p):
+        """Q P with the per-atom onsite metric, src/plane_wave_operator.cpp:381-389."""
+        p = np.asfortranarray(p, dtype=np.complex128)
+        qp = np.zeros_like(p, order="F")
+        _check(lib().sdft_hip_operator_projector_metric(self.handle, _p(p), ct.c_size_t(p.shape[1]), _p(qp)))
+        return qp
+
+    def density(self, coefficients, occupations, volume, support_hkl):
+        """Smooth density Fourier coefficients at support_hkl (m x 3 ints),
+        src/density_functionals.cpp:406-460."""
+        c = np.asfortranarray(coefficients, dtype=np.complex128)
+        occ = np.ascontiguousarray(occupations, dtype=np.float64)
+        keys = np.ascontiguousarray(support_hkl, dtype=np.int32)
+        out = np.zeros(len(keys), dtype=np.complex128)
+        _check(lib().sdft_hip_operator_density(self.handle, _p(c), ct.c_size_t(c.shape[1]), _p(occ),
+                                               ct.c_double(volume), ct.c_size_t(len(keys)), _p(keys, _ip), _p(out)))
+        return out
+
+    def save_snapshot(self, path, roots, iterations, tolerance, initial=None):
+        init = None if initial is None else np.asfortranarray(initial[:, :roots], dtype=np.complex128)
+        _check(lib().sdft_hip_snapshot_save(self.handle, os.fsencode(path), ct.c_size_t(roots),
+                                            ct.c_size_t(iterations), ct.c_double(tolerance), _p(init)))
+
+    def _describe(self):
         shape_out = (ct.c_int * 3)()
         count = ct.c_long()
         npj = ct.c_long()
@@ -293,9 +391,9 @@ class Operator:
 
     def davidson(self, roots, max_iterations=256, tolerance=1e-12, initial=None, subspace_factor=8,
                  search_precision=0, search_handoff=1e-8, stagnation_window=3, handoff_restart=1, verbose=0,
-                 timing=0, want_vectors=True):
+                 timing=0, want_vectors=True, band_group_size=0):
         p = DavidsonParams(roots, max_iterations, tolerance, subspace_factor, search_precision, search_handoff,
-                           stagnation_window, handoff_restart, verbose, timing)
+                           stagnation_window, handoff_restart, verbose, timing, band_group_size)
         ev = np.zeros(roots)
         res = np.zeros(roots)
         vec = np.zeros((self.n, roots), dtype=np.complex128, order="F") if want_vectors else None
@@ -309,4 +407,5 @@ class Operator:
                     converged=bool(r.converged), h_columns_search=r.h_columns_search,
                     h_columns_exact=r.h_columns_exact, handoff_iteration=r.handoff_iteration,
                     seconds_total=r.seconds_total, seconds_apply=r.seconds_apply, seconds_gemm=r.seconds_gemm,
-                    seconds_ritz=r.seconds_ritz, seconds_ortho=r.seconds_ortho)
+                    seconds_ritz=r.seconds_ritz, seconds_ortho=r.seconds_ortho,
+                    cholesky_blocks=r.cholesky_blocks, rr_cholesky=r.rr_cholesky)

@@ -1,0 +1,136 @@
+"""Solver replay on the GPU: the harness of src/solver_replay_cli.cpp.
+
+    python -m sdft_amd.replay snapshot.pwop report.json [max_iterations] [warm_snapshot.pwop]
+
+Loads a SDFTPWO1/2/3 snapshot into a device operator, runs the device block
+Davidson with the stored request (roots, iteration limit, tolerance, warm
+block), applies fresh FP64 H and S to the returned vectors, and writes the
+report with the reference's schema ("sparse-dft-solver-replay-1"). Passed
+means: the solver converged, every fresh residual ||H x - e S x|| <= tolerance
+and every |x^H S x - 1| <= 1e-12. Exit status 0 passed, 2 not passed, 1 error
+(message prefixed SDFT_REPLAY_ERROR). A `<snapshot>.guards` file (SDFTSG01,
+src/solver_search_state.hpp) is validated against the snapshot's SHA-256; as
+in the reference's default controls (spectral-guard recycling off) it does not
+seed the solve.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import native
+
+SCHEMA = "sparse-dft-solver-replay-1"
+
+
+def file_sha256(path):
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 20), b""):
+            h.update(chunk)
+    return h.hexdigest()
+
+
+def load_search_guards(path, parent_sha, dimension, maximum_columns):
+    """SDFTSG01: magic, 64-byte hex SHA-256 of the parent snapshot, uint64
+    {dimension, columns}, complex128 guards (src/solver_search_state.hpp:33-60)."""
+    size = os.path.getsize(path)
+    if size < 88:
+        raise RuntimeError("truncated diagnostic search header")
+    with open(path, "rb") as f:
+        magic, sha = f.read(8), f.read(64)
+        dim, cols = (int(v) for v in np.frombuffer(f.read(16), dtype="<u8"))
+        if (magic != b"SDFTSG01" or sha.decode("ascii", "replace") != parent_sha or dim != dimension
+                or cols > maximum_columns or size != 88 + dimension * cols * 16):
+            raise RuntimeError("diagnostic search state identity/dimensions/size mismatch")
+        guards = np.frombuffer(f.read(), dtype="<c16").reshape((dimension, cols), order="F")
+    if not np.all(np.isfinite(guards)):
+        raise RuntimeError("nonfinite diagnostic search payload")
+    return guards
+
+
+def save_search_guards(path, parent_sha, guards):
+    guards = np.asarray(guards, np.complex128)
+    if len(parent_sha) != 64 or guards.ndim != 2 or os.path.exists(path) or not np.all(np.isfinite(guards)):
+        raise ValueError("invalid diagnostic search state or existing output")
+    with open(path, "wb") as f:
+        f.write(b"SDFTSG01" + parent_sha.encode("ascii") + np.array(guards.shape, dtype="<u8").tobytes()
+                + guards.ravel(order="F").tobytes())
+
+
+def fresh_certificate(op, eigenvalues, vectors):
+    """Fresh FP64 H x and S x; residual norms and S-normalisation errors
+    accumulated in extended precision (src/solver_replay_cli.cpp:30-45)."""
+    hx, sx = op.apply(vectors, "pair", precision=0)
+    r = hx - sx * eigenvalues
+    ld = np.longdouble
+    residual = np.sqrt((r.real.astype(ld) ** 2 + r.imag.astype(ld) ** 2).sum(axis=0)).astype(np.float64)
+    metric = (vectors.real.astype(ld) * sx.real + vectors.imag.astype(ld) * sx.imag).sum(axis=0)
+    return residual, np.abs(metric.astype(np.float64) - 1.0)
+
+
+def replay(snapshot, report, max_iterations=None, warm_snapshot=None, ctx=None, search_precision=0):
+    if os.path.exists(report):
+        raise RuntimeError("refusing to overwrite replay report")
+    info = native.snapshot_info(snapshot)
+    iterations = info["iterations"] if max_iterations is None else int(max_iterations)
+    if not 0 < iterations <= 1000000:
+        raise ValueError("invalid replay iteration limit")
+    ctx = ctx or native.Context(0)
+    start = time.perf_counter()
+    op, initial = native.Operator.from_snapshot(ctx, snapshot)
+    guard_path = snapshot + ".guards"
+    guards = None
+    if os.path.exists(guard_path):
+        guards = load_search_guards(guard_path, file_sha256(snapshot), info["n"], info["roots"])
+    roots, tol = info["roots"], info["tolerance"]
+    r = op.davidson(roots, iterations, tol, initial, search_precision=search_precision)
+    residual, normalization = fresh_certificate(op, r["eigenvalues"], r["eigenvectors"])
+    passed = bool(r["converged"] and residual.max() <= tol and normalization.max() <= 1e-12)
+    seconds = time.perf_counter() - start
+    out = {"schema_version": SCHEMA, "scientific_scf_result": False, "passed": passed, "dimension": info["n"],
+           "roots": roots, "stored_max_iterations": info["iterations"], "max_iterations": iterations,
+           "iterations": r["iterations"], "tolerance": tol, "solver_converged": bool(r["converged"]),
+           "fresh_max_residual": float(residual.max()), "s_normalization_error": float(normalization.max()),
+           "seconds": seconds, "residuals": [float(v) for v in r["residual_norms"]],
+           "eigenvalues_hartree": [float(v) for v in r["eigenvalues"]]}
+    with open(report, "w") as f:
+        f.write(json.dumps(out) + "\n")
+    perf = {"device": ctx.device_name(), "search_precision": search_precision,
+            "guard_columns_validated": 0 if guards is None else int(guards.shape[1]),
+            **{k: r[k] for k in ("seconds_total", "seconds_apply", "seconds_gemm", "seconds_ritz", "seconds_ortho",
+                                 "h_columns_search", "h_columns_exact", "cholesky_blocks", "rr_cholesky")}}
+    with open(report + ".performance.json", "w") as f:
+        f.write(json.dumps(perf) + "\n")
+    if warm_snapshot:
+        op.save_snapshot(warm_snapshot, roots, iterations, tol, r["eigenvectors"])
+    return out
+
+
+def main(argv=None):
+    argv = sys.argv[1:] if argv is None else argv
+    try:
+        if not 2 <= len(argv) <= 4:
+            raise ValueError("usage: python -m sdft_amd.replay snapshot.pwop report.json [max_iterations] "
+                             "[warm_snapshot.pwop]")
+        limit = None
+        if len(argv) >= 3:
+            if not argv[2].isdigit():
+                raise ValueError("invalid replay iteration limit")
+            limit = int(argv[2])
+        out = replay(argv[0], argv[1], limit, argv[3] if len(argv) == 4 else None)
+        print(f"solver_replay {'PASS' if out['passed'] else 'FAIL'} residual={out['fresh_max_residual']} "
+              f"tolerance={out['tolerance']} iterations={out['iterations']}")
+        return 0 if out["passed"] else 2
+    except Exception as e:  # noqa: BLE001 - the harness reports every failure the same way
+        print(f"SDFT_REPLAY_ERROR: {e}", file=sys.stderr)
+        return 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,95 @@
+"""Serving workload: certified eigensolves arriving as a Poisson stream.
+
+Requests are drawn uniformly from diamond Si cells of 8, 16, 32 and 64 atoms
+(1x1x1, 2x1x1, 2x2x1, 2x2x2 supercells of a = 10.26 bohr) at a plane-wave
+cutoff drawn from {10, 11.25, 12.5, 13.75, 15} Ha, with 2.5 bands per atom
+(20 / 40 / 80 / 160 roots) and a seeded warm start. Each request's input is a
+distinct synthetic operator (V(G) perturbation seed per input); to keep the
+host-side generation bounded, `pool` inputs are generated once and requests
+cycle through them. A request's latency runs from its arrival to its
+certified result and includes the device set-up of its operator (upload,
+V(r) transform, projector upload) and the solve.
+
+The arrival rate defaults to 0.7 x the saturation throughput measured by a
+closed-loop pass over the same pool (all requests queued at once).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import serve
+from . import synthetic as syn
+
+CELLS = [(1, 1, 1), (2, 1, 1), (2, 2, 1), (2, 2, 2)]
+ECUTS = [10.0, 11.25, 12.5, 13.75, 15.0]
+CONFIG = 4
+
+
+def request_pool(pool, seed=4):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(pool):
+        reps = CELLS[i % len(CELLS)]
+        ecut = float(rng.choice(ECUTS))
+        atoms = 8 * int(np.prod(reps))
+        cell = syn.diamond_cell(10.26, reps)
+        prob = syn.config_problem(0, cell=cell, ecut=ecut, shape=(0, 0, 0), perturbation=i + 1)
+        roots = atoms * 5 // 2
+        out.append(dict(prob=prob, roots=roots, init=syn.initial_psi(prob, roots, CONFIG, extra=i), atoms=atoms,
+                        ecut=ecut))
+    return out
+
+
+def run(svc, pool, requests, rate=None, seed=5, davidson=None):
+    """Submit `requests` solves (cycling through `pool`); rate None = all at
+    t = 0 (closed loop), else Poisson arrivals at `rate` per second.
+    Returns per-request records."""
+    rng = np.random.default_rng(seed)
+    davidson = dict(davidson or {})
+    t0 = time.perf_counter()
+    arrivals = np.zeros(requests) if rate is None else np.cumsum(rng.exponential(1.0 / rate, requests))
+    futs = []
+    for i in range(requests):
+        target = t0 + arrivals[i]
+        delay = target - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        job = pool[i % len(pool)]
+        futs.append((job, svc.submit(job["prob"], job["roots"], job["init"], arrival=target, **davidson)))
+    records = []
+    for job, f in futs:
+        r = f.result()
+        if not r["converged"]:
+            raise RuntimeError("a serving request did not certify")
+        records.append(dict(atoms=job["atoms"], ecut=job["ecut"], roots=job["roots"], latency=r["finish"] - r["arrival"],
+                            service=r["finish"] - r["start"], finish=r["finish"], arrival=r["arrival"],
+                            max_residual=float(r["residual_norms"].max())))
+    return records, time.perf_counter() - t0
+
+
+def summary(records, wall):
+    lat = np.array([r["latency"] for r in records])
+    pairs = sum(r["roots"] for r in records)
+    return dict(requests=len(records), p50_latency_s=float(np.percentile(lat, 50)),
+                p99_latency_s=float(np.percentile(lat, 99)), mean_latency_s=float(lat.mean()),
+                throughput_requests_per_s=len(records) / wall, certified_eigenpairs_per_s=pairs / wall,
+                max_certified_residual_ha=max(r["max_residual"] for r in records))
+
+
+def serving(requests=200, pool=20, workers=4, load=0.7, davidson=None):
+    jobs = request_pool(pool)
+    svc = serve.SolveService(workers=workers)
+    try:
+        run(svc, jobs[:len(CELLS)], len(CELLS), davidson=davidson)  # warm-up: one of each size
+        closed, wall_closed = run(svc, jobs, max(len(jobs), requests // 4), davidson=davidson)
+        saturation = len(closed) / wall_closed
+        rate = load * saturation
+        opened, wall_open = run(svc, jobs, requests, rate=rate, davidson=davidson)
+    finally:
+        svc.close()
+    out = summary(opened, wall_open)
+    out.update(saturation_requests_per_s=saturation, arrival_rate_per_s=rate, load=load, workers=workers,
+               pool=pool, mix="Si8/16/32/64 uniform, Ecut in {10..15} Ha, 2.5 bands/atom")
+    return out

@@ -72,6 +72,13 @@ def main():
     assert r8["converged"]
     out["si8_cold8_eigenvalues"] = r8["eigenvalues"]
     out["si8_cold8_iterations"] = np.array(r8["iterations"])
+    # reference bounded band-group solver (src/band_group_solver.cpp), groups of 8 and 5
+    init16 = syn.initial_psi(prob, 16, 0)
+    for g in (8, 5):
+        rg = pyref.davidson(op, 16, 256, 1e-12, init16, band_group_size=g)
+        assert rg["converged"]
+        out[f"si8_bandgroups{g}_eigenvalues"] = rg["eigenvalues"]
+        out[f"si8_bandgroups{g}_max_residual"] = np.array(rg["residual_norms"].max())
     # compensated product vectors (src/paw_compensated_product.hpp)
     rng = np.random.default_rng(20261202)
     a = rng.standard_normal((18, 18)) + 1j * rng.standard_normal((18, 18))

@@ -70,3 +70,15 @@ def test_oracle_davidson_matches_reference_davidson(si8_oracle, si8):
     r8 = si8_oracle.davidson(8, 256, 1e-12)
     assert np.abs(r8["eigenvalues"] - GOLDEN["si8_cold8_eigenvalues"]).max() < 1e-12
     assert r8["iterations"] == int(GOLDEN["si8_cold8_iterations"])
+
+
+@pytest.mark.parametrize("group", [8, 5])
+def test_reference_band_groups_against_dense(si8, group):
+    """The reference's bounded band-group solver (src/band_group_solver.cpp,
+    SDFT_BAND_GROUP_SIZE = 8 and 5) on the Si8 operator: its certified
+    eigenvalues are the dense generalized eigenvalues."""
+    import scipy.linalg as sl
+    h, s = syn.dense_operator(si8)
+    ref = sl.eigh(h, s, eigvals_only=True, subset_by_index=[0, 15])
+    assert np.abs(GOLDEN[f"si8_bandgroups{group}_eigenvalues"] - ref).max() < 1e-10
+    assert GOLDEN[f"si8_bandgroups{group}_max_residual"] <= 1e-12

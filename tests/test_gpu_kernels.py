@@ -174,6 +174,27 @@ def test_device_operator_apply_si8(si8_device, si8_oracle, si8, which):
         assert np.abs(g32 - r).max() / scale < 2e-5
 
 
+def test_kernel_clock_counts_launches_and_work(si8_device, si8):
+    """The event clock sees exactly the GEMM and local-H launches of one pair
+    apply and credits each with its flops / algorithmic bytes."""
+    ctx = si8_device.ctx
+    x = syn.initial_psi(si8, 13, 0)
+    ctx.kernel_clock(True)
+    try:
+        si8_device.apply(x, "pair", precision=1)
+        st = ctx.kernel_clock_stats()
+    finally:
+        ctx.kernel_clock(False)
+    n, npr, m, count = si8.n, si8.np_total, 13, int(np.prod(si8_device.shape))
+    # P = B X, H += B^H (DP), S = B^H (QP) + (1 + reg) X: three FP32-arithmetic GEMMs
+    assert st["gemm_fp32"]["launches"] == 3
+    assert st["gemm_fp32"]["work"] == pytest.approx(3 * 8.0 * n * npr * m)
+    assert st["local_fp32"]["launches"] == 1
+    assert st["local_fp32"]["work"] == pytest.approx(m * (2 * n * 8 + 4 * count * 8) + count * 4 + n * 8)
+    assert st["gemm_fp64"]["launches"] == 0 and st["local_fp64"]["launches"] == 0
+    assert all(v["ms"] > 0 for v in st.values() if v["launches"])
+
+
 def test_device_operator_smoke_geometry(hip_ctx):
     prob, hd, sd = smoke_problem()
     op = native.Operator(hip_ctx, prob, shape=(0, 0, 0))

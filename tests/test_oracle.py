@@ -294,3 +294,21 @@ def test_blas_bound_oracle_matches_plain_loops(si8, si8_oracle):
     assert np.abs(bound[0] - plain[0]).max() < 1e-12
     assert np.abs(bound[1] - plain[1]).max() < 1e-11
     assert np.abs(bound[2]["eigenvalues"] - plain[2]["eigenvalues"]).max() < 1e-9
+
+
+def test_density_oracle_against_brute_force():
+    """The FFT restatement of plane_wave_density_fft against the direct
+    autocorrelation sum over G1 - G2 on a tiny basis (pins the sign of G,
+    the 1/N and the occupation/volume weights)."""
+    rng = np.random.default_rng(9)
+    hkl = np.array([[h, k, l] for h in range(-1, 2) for k in range(-1, 2) for l in range(-1, 2)
+                    if h * h + k * k + l * l <= 2])
+    c = rng.standard_normal((len(hkl), 3)) + 1j * rng.standard_normal((len(hkl), 3))
+    occ = np.array([2.0, 1.25, 0.0])
+    support = np.array([[h, k, l] for h in range(-3, 4) for k in range(-2, 3) for l in range(-2, 3)])
+    a = pyoracle.density_fft(hkl, c, occ, 17.5, support)
+    b = pyoracle.density_brute_force(hkl, c, occ, 17.5, support)
+    assert np.abs(a - b).max() < 1e-13 * np.abs(b).max()
+    # rho(-G) = conj(rho(G)) for a real density; rho(0) = sum occ |c|^2 / volume
+    zero = np.where((support == 0).all(axis=1))[0][0]
+    assert abs(a[zero] - (occ * (np.abs(c) ** 2).sum(axis=0)).sum() / 17.5) < 1e-12
