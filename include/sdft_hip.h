@@ -181,6 +181,27 @@ typedef struct {
 int sdft_hip_operator_project(sdft_hip_operator* op, const double* x, size_t columns, double* p, int precision);
 int sdft_hip_operator_projector_metric(sdft_hip_operator* op, const double* p, size_t columns, double* qp);
 
+/* Real-space projector search (src/plane_wave_operator.cpp:104-163; the
+ * reference's SDFT_PROJECTOR_SPACE=real-search with SDFT_PROJECTOR_RADIUS_FACTOR).
+ * factor in [1,2] enables the approximate operator for FP32 search
+ * applications (0 disables). Per atom, with geometry (lattice: 3 rows of cell
+ * vectors in bohr, fractional: atoms x 3, radius: atoms, as the reference's
+ * atom_geometry) and radius > 0, the grid points within factor * radius of the
+ * atom (minimum image) are kept; otherwise (geometry NULL, as for snapshots)
+ * those where the projector amplitude exceeds 10^(-2-5(factor-1)) of its
+ * maximum. *points: retained points; *retained: kept fraction of the
+ * projector power (either may be NULL). */
+int sdft_hip_operator_real_space_search(sdft_hip_operator* op, double factor, const double* lattice,
+                                        const double* fractional, const double* radius, size_t* points,
+                                        double* retained);
+/* H (which 0) or S (which 1) through the real-space projectors, host pointers;
+ * precision 0 FP64, 1 FP32. */
+int sdft_hip_operator_apply_real_space(sdft_hip_operator* op, int which, const double* x, size_t columns, double* y,
+                                       int precision);
+/* Retained points as natural grid indices, atom by atom (total entries), and
+ * the count per atom. */
+int sdft_hip_operator_real_space_points(sdft_hip_operator* op, long* natural, long* per_atom);
+
 /* plane_wave_density_fft (src/density_functionals.cpp:406-460): Fourier
  * coefficients (nsupport complex) at the Miller triples support_hkl of
  * rho(r) = sum_b occupations[b] / volume |psi_b(r)|^2, psi_b the band

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dft_constants.hpp"
+#include "slot_order.hpp"
 
 namespace sdft_hip {
 
@@ -248,12 +249,6 @@ __device__ __forceinline__ void lds_fft_from_slots(cx<T>* base, const LineGeom& 
   __syncthreads();
   lds_strided_pass<L1, L2, S, false, T>(base, g, tw, tid, nt);
   __syncthreads();
-}
-
-// Position (natural index) held by slot s for a length L1 * L2 line.
-__host__ __device__ inline int slot_position(int s, int L1, int L2) {
-  const int a = s % L1, b = s / L1;
-  return b + L2 * a;
 }
 
 }  // namespace sdft_hip

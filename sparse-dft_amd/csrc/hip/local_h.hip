@@ -95,30 +95,37 @@ __global__ __launch_bounds__(kPlaneThreads) void k_planes(const cx<T>* __restric
   const int zs = blockIdx.x;
   const long col = blockIdx.y;
   const int np = n0 * n1;
-  for (int e = tid; e < np; e += nt) plane[e] = {T(0), T(0)};
-  __syncthreads();
-  const cx<T>* in = t1 + (col * n2 + zs) * (long)nsticks;
-  for (int s = tid; s < nsticks; s += nt) plane[stick_xy[s]] = in[s];
-  __syncthreads();
   // y lines: one per x, contiguous elements (two ranges of active x)
   const LineGeom gy0{xl0, n1, 1, false}, gy1{xl1, n1, 1, false};
   // x lines: one per y, elements strided by n1
   const LineGeom gx{n1, 1, n1, true};
-  if (xl0) line_fft_dispatch<T, +1, true>(n1, plane + xs0 * n1, gy0, tw_y, tid, nt);
-  if (xl1) line_fft_dispatch<T, +1, true>(n1, plane + xs1 * n1, gy1, tw_y, tid, nt);
-  line_fft_dispatch<T, +1, true>(n0, plane, gx, tw_x, tid, nt);
-  if (mode == 1) {
-    cx<T>* out = t2 + (col * n2 + zs) * (long)np;
-    for (int e = tid; e < np; e += nt) out[e] = plane[e];
-    return;
+  if (mode == 2) {
+    // start from a full slot-order plane (real-space search images)
+    const cx<T>* inp = t1 + (col * n2 + zs) * (long)np;
+    for (int e = tid; e < np; e += nt) plane[e] = inp[e];
+    __syncthreads();
+  } else {
+    for (int e = tid; e < np; e += nt) plane[e] = {T(0), T(0)};
+    __syncthreads();
+    const cx<T>* in = t1 + (col * n2 + zs) * (long)nsticks;
+    for (int s = tid; s < nsticks; s += nt) plane[stick_xy[s]] = in[s];
+    __syncthreads();
+    if (xl0) line_fft_dispatch<T, +1, true>(n1, plane + xs0 * n1, gy0, tw_y, tid, nt);
+    if (xl1) line_fft_dispatch<T, +1, true>(n1, plane + xs1 * n1, gy1, tw_y, tid, nt);
+    line_fft_dispatch<T, +1, true>(n0, plane, gx, tw_x, tid, nt);
+    if (mode == 1) {
+      cx<T>* out = t2 + (col * n2 + zs) * (long)np;
+      for (int e = tid; e < np; e += nt) out[e] = plane[e];
+      return;
+    }
+    const T* v = v_slots + (long)zs * np;
+    for (int e = tid; e < np; e += nt) {
+      const T w = v[e];
+      plane[e].x *= w;
+      plane[e].y *= w;
+    }
+    __syncthreads();
   }
-  const T* v = v_slots + (long)zs * np;
-  for (int e = tid; e < np; e += nt) {
-    const T w = v[e];
-    plane[e].x *= w;
-    plane[e].y *= w;
-  }
-  __syncthreads();
   line_fft_dispatch<T, -1, false>(n0, plane, gx, tw_x, tid, nt);
   if (xl0) line_fft_dispatch<T, -1, false>(n1, plane + xs0 * n1, gy0, tw_y, tid, nt);
   if (xl1) line_fft_dispatch<T, -1, false>(n1, plane + xs1 * n1, gy1, tw_y, tid, nt);
@@ -369,6 +376,56 @@ hipError_t launch_backward_planes(const GridPlanDev& p, const double* x, long ld
                      p.xlines[0], p.xstart[1], p.xlines[1]);
   return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_backward_full(const GridPlanDev& p, const double* x, long ldx, long columns, void* t1, void* t2,
+                                hipStream_t stream) {
+  if (columns <= 0) return hipSuccess;
+  if (size_t(p.n0) * p.n1 * sizeof(cx<T>) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
+  const int spb = sticks_per_block(p.n2);
+  const dim3 ga((p.nsticks + spb - 1) / spb, (unsigned)columns);
+  const size_t lds_lines = size_t(spb) * p.n2 * sizeof(cx<T>);
+  const cx<T>* twx = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[0] : p.tw32[0]);
+  const cx<T>* twy = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[1] : p.tw32[1]);
+  const cx<T>* twz = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[2] : p.tw32[2]);
+  hipLaunchKernelGGL(k_sticks_backward<T>, ga, dim3(256), lds_lines, stream, x, ldx, p.stick_g, p.nsticks, p.n2, spb,
+                     twz, reinterpret_cast<cx<T>*>(t1));
+  const size_t lds_plane = size_t(p.n0) * p.n1 * sizeof(cx<T>);
+  hipLaunchKernelGGL(k_planes<T>, dim3(p.n2, (unsigned)columns), dim3(kPlaneThreads), lds_plane, stream,
+                     reinterpret_cast<const cx<T>*>(t1), reinterpret_cast<cx<T>*>(t2), p.stick_xy, p.nsticks, p.n0,
+                     p.n1, p.n2, twx, twy, (const T*)nullptr, 1, p.xstart[0], p.xlines[0], p.xstart[1], p.xlines[1]);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_forward_planes(const GridPlanDev& p, const void* planes, long columns, const double* x, long ldx,
+                                 const double* diag, double* y, long ldy, void* t_sticks, hipStream_t stream) {
+  if (columns <= 0) return hipSuccess;
+  if (size_t(p.n0) * p.n1 * sizeof(cx<T>) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
+  const cx<T>* twx = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[0] : p.tw32[0]);
+  const cx<T>* twy = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[1] : p.tw32[1]);
+  const cx<T>* twz = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[2] : p.tw32[2]);
+  const size_t lds_plane = size_t(p.n0) * p.n1 * sizeof(cx<T>);
+  hipLaunchKernelGGL(k_planes<T>, dim3(p.n2, (unsigned)columns), dim3(kPlaneThreads), lds_plane, stream,
+                     reinterpret_cast<const cx<T>*>(planes), reinterpret_cast<cx<T>*>(t_sticks), p.stick_xy,
+                     p.nsticks, p.n0, p.n1, p.n2, twx, twy, (const T*)nullptr, 2, p.xstart[0], p.xlines[0],
+                     p.xstart[1], p.xlines[1]);
+  const int spb = sticks_per_block(p.n2);
+  const dim3 ga((p.nsticks + spb - 1) / spb, (unsigned)columns);
+  const size_t lds_lines = size_t(spb) * p.n2 * sizeof(cx<T>);
+  hipLaunchKernelGGL(k_sticks_forward<T>, ga, dim3(256), lds_lines, stream, reinterpret_cast<const cx<T>*>(t_sticks),
+                     p.stick_g, p.nsticks, p.n2, spb, twz, x, ldx, diag, y, ldy, 0.0, double(p.count));
+  return hipGetLastError();
+}
+
+template hipError_t launch_backward_full<double>(const GridPlanDev&, const double*, long, long, void*, void*,
+                                                 hipStream_t);
+template hipError_t launch_backward_full<float>(const GridPlanDev&, const double*, long, long, void*, void*,
+                                                hipStream_t);
+template hipError_t launch_forward_planes<double>(const GridPlanDev&, const void*, long, const double*, long,
+                                                  const double*, double*, long, void*, hipStream_t);
+template hipError_t launch_forward_planes<float>(const GridPlanDev&, const void*, long, const double*, long,
+                                                 const double*, double*, long, void*, hipStream_t);
 
 hipError_t launch_density_accumulate(const void* planes, long count, int columns, const double* weights,
                                      double* density, hipStream_t stream) {

@@ -42,6 +42,15 @@ int max_lds_bytes();
 // a real field).
 hipError_t launch_backward_planes(const GridPlanDev& p, const double* x, long ldx, long columns, void* t1, void* t2,
                                   hipStream_t stream);
+// Backward transform of columns into full slot-order planes at precision T
+// (t2: columns x count), and the forward transform from full slot-order planes
+// onto the sphere: y = diag .* x + fwd(planes)[sphere] / N (t_sticks: stick buffer).
+template <typename T>
+hipError_t launch_backward_full(const GridPlanDev& p, const double* x, long ldx, long columns, void* t1, void* t2,
+                                hipStream_t stream);
+template <typename T>
+hipError_t launch_forward_planes(const GridPlanDev& p, const void* planes, long columns, const double* x, long ldx,
+                                 const double* diag, double* y, long ldy, void* t_sticks, hipStream_t stream);
 hipError_t launch_density_accumulate(const void* planes, long count, int columns, const double* weights,
                                      double* density, hipStream_t stream);
 hipError_t launch_slots_to_natural_complex(const GridPlanDev& p, const double* v_slots, double* grid,

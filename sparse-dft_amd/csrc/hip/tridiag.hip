@@ -14,9 +14,10 @@
 //   4. w = tau p - (tau/2)(tau p)^H v v, the rank-2 update of its rows, and the
 //      next column for all rows from the published A_{i,k+1}.
 // The published vectors are double-buffered by step parity, so one barrier per
-// column suffices. The barrier spins on an agent-scope counter with a bounded
-// wait: on expiry it raises an abort flag that every workgroup observes, and
-// all workgroups leave the kernel.
+// column suffices. Each workgroup raises its own arrival flag (one cache line
+// per workgroup, no read-modify-write contention) and polls all flags with one
+// wave, with a bounded wait: on expiry it raises an abort flag that every
+// workgroup observes, and all workgroups leave the kernel.
 #include <hip/hip_runtime.h>
 
 #include "tridiag.hpp"
@@ -75,31 +76,48 @@ __device__ Z block_zsum(Z v, double* scratch) {
   return t;
 }
 
-// Grid barrier number `k` (0-based): returns false when the wait expired or
-// another workgroup gave up.
-__device__ bool grid_barrier(unsigned* counter, int* abort_flag, unsigned target, int* ok_shared) {
-  __threadfence();
+// Grid barrier with publication: thread 0 writes the workgroup's staged
+// values (count entries of `src`, LDS, to dst[index[j]]), releases them at agent
+// scope and raises its own arrival flag to `step` (one flag per workgroup, each
+// on its own 64-byte line, so arrivals never contend); the lanes of wave 0 then
+// poll every workgroup's flag in parallel and acquire. Returns false when the
+// wait expired or another workgroup gave up.
+constexpr int kFlagStride = 16;  // unsigned ints per flag line
+__device__ bool publish_and_wait(unsigned* flags, int* abort_flag, unsigned step, int* ok_shared, int count,
+                                 const int* index, const Z* src_p, const Z* src_c, Z* dst_p, Z* dst_c) {
   __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    int ok = 1;
-    for (long spin = 0;; ++spin) {
-      if (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
-      if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        ok = 0;
-        break;
-      }
-      if (spin > kSpinLimit) {
-        __hip_atomic_fetch_or(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ok = 0;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    for (int j = 0; j < count; ++j) {
+      dst_p[index[j]] = src_p[j];
+      dst_c[index[j]] = src_c[j];
     }
-    *ok_shared = ok;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(flags + blockIdx.x * kFlagStride, step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (tid < 64) {
+    int ok = 1;
+    for (int q = tid; q < int(gridDim.x) && ok; q += 64) {
+      for (long spin = 0;; ++spin) {
+        if (__hip_atomic_load(flags + q * kFlagStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= step) break;
+        if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          ok = 0;
+          break;
+        }
+        if (spin > kSpinLimit) {
+          __hip_atomic_fetch_or(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    // every lane of wave 0 must have seen its flags
+    const unsigned long long all_ok = __ballot(ok);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (tid == 0) *ok_shared = all_ok == ~0ull ? 1 : 0;
   }
   __syncthreads();
-  __threadfence();
   return *ok_shared != 0;
 }
 
@@ -107,14 +125,17 @@ __global__ __launch_bounds__(kThreads) void k_hetrd_lds(const Z* __restrict__ a,
                                                         Z* __restrict__ refl, long ldr, double* __restrict__ d,
                                                         double* __restrict__ e, Z* __restrict__ tau,
                                                         Z* __restrict__ pub_p, Z* __restrict__ pub_c,
-                                                        unsigned* __restrict__ counter, int* __restrict__ abort_flag) {
+                                                        unsigned* __restrict__ flags, int* __restrict__ abort_flag) {
   extern __shared__ double lds_raw[];
   Z* rows = reinterpret_cast<Z*>(lds_raw);  // rpw x n
   Z* v = rows + size_t(rpw) * n;            // reflector (trailing part), n
   Z* w = v + n;                             // p, then w
   Z* col = w + n;                           // current column, n
   double* scratch = reinterpret_cast<double*>(col + n);  // 2 * kWaves
-  __shared__ int ok_shared;
+  Z* stage_p = reinterpret_cast<Z*>(scratch + 2 * kWaves);  // rpw staged p_i
+  Z* stage_c = stage_p + rpw;                               // rpw staged A_{i,k+1}
+  int* stage_i = reinterpret_cast<int*>(stage_c + rpw);     // rpw row indices
+  __shared__ int ok_shared, staged;
   __shared__ Z sh_a2;
 
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
@@ -157,28 +178,38 @@ __global__ __launch_bounds__(kThreads) void k_hetrd_lds(const Z* __restrict__ a,
     if (g == 0)
       for (int j = 2 + tid; j <= m; j += kThreads) refl[(k + j) + size_t(k) * ldr] = v[j - 1];
 
-    // ---- 2. p_i = sum_j A_ij v_j on own trailing rows; publish p_i, A_{i,k+1}
+    // ---- 2. p_i = sum_j A_ij v_j on own trailing rows, staged in LDS with the
+    // pre-update A_{i,k+1}; the active rows are r >= first (rows g + r G > k)
     Z* pp = pub_p + size_t(k & 1) * n;
     Z* pc = pub_c + size_t(k & 1) * n;
+    const int first = k + 1 - g <= 0 ? 0 : (k + 1 - g + G - 1) / G;
     {
       const int lane = tid & 63;
-      for (int r = tid >> 6; r < rpw; r += kWaves) {  // one wave per row, no block barrier
+      for (int r = first + (tid >> 6); r < rpw; r += kWaves) {  // one wave per row, no block barrier
         const int i = g + r * G;
         if (i >= n) break;
-        if (i <= k) continue;
         const Z* row = rows + size_t(r) * n + (k + 1);
         Z acc{0., 0.};
         for (int j = lane; j < m; j += 64) acc = zadd(acc, zmul(row[j], v[j]));
         acc.x = wave_sum(acc.x);
         acc.y = wave_sum(acc.y);
         if (lane == 0) {
-          pp[i] = acc;
-          pc[i] = row[0];
+          stage_p[r - first] = acc;
+          stage_c[r - first] = row[0];
+          stage_i[r - first] = i;
         }
       }
+      if (tid == 0) {
+        int cnt = 0;
+        for (int r = first; r < rpw && g + r * G < n; ++r) ++cnt;
+        staged = cnt;
+      }
     }
-    // ---- 3. grid barrier
-    if (!grid_barrier(counter, abort_flag, unsigned(G) * unsigned(k + 1), &ok_shared)) return;
+    // ---- 3. publication and grid barrier
+    __syncthreads();
+    if (!publish_and_wait(flags, abort_flag, unsigned(k + 1), &ok_shared, staged, stage_i, stage_p,
+                          stage_c, pp, pc))
+      return;
     // ---- 4. w, rank-2 update of own rows, next column for all rows
     for (int j = tid; j < m; j += kThreads) {
       w[j] = pp[k + 1 + j];
@@ -230,7 +261,10 @@ int device_cus() {
   return cus;
 }
 
-size_t lds_bytes(int n, int rpw) { return (size_t(rpw) * n + 3 * size_t(n)) * sizeof(Z) + 2 * kWaves * sizeof(double); }
+size_t lds_bytes(int n, int rpw) {
+  return (size_t(rpw) * n + 3 * size_t(n) + 2 * size_t(rpw)) * sizeof(Z) + 2 * kWaves * sizeof(double) +
+         size_t(rpw) * sizeof(int);
+}
 }  // namespace
 
 bool hetrd_plan(int n, int* rows_per_group, int* groups) {
@@ -249,7 +283,11 @@ bool hetrd_plan(int n, int* rows_per_group, int* groups) {
   return true;
 }
 
-size_t hetrd_workspace_bytes(int n) { return 4 * size_t(n) * sizeof(Z) + 64; }
+// published vectors (4 n complex), then the abort flag line, then one flag line
+// per workgroup (at most kHetrdMaxOrder workgroups)
+size_t hetrd_workspace_bytes(int n) {
+  return 4 * size_t(n) * sizeof(Z) + size_t(kFlagStride) * sizeof(unsigned) * (1 + size_t(kHetrdMaxOrder));
+}
 
 hipError_t launch_hetrd(const double* a, long lda, int n, double* refl, long ldr, double* d, double* e, double* tau,
                         void* workspace, hipStream_t s) {
@@ -261,22 +299,22 @@ hipError_t launch_hetrd(const double* a, long lda, int n, double* refl, long ldr
   }
   Z* pub_p = reinterpret_cast<Z*>(workspace);
   Z* pub_c = pub_p + 2 * size_t(n);
-  unsigned* counter = reinterpret_cast<unsigned*>(pub_c + 2 * size_t(n));
-  int* abort_flag = reinterpret_cast<int*>(counter + 1);
-  hipError_t err = hipMemsetAsync(counter, 0, 2 * sizeof(unsigned), s);
+  int* abort_flag = reinterpret_cast<int*>(pub_c + 2 * size_t(n));
+  unsigned* flags = reinterpret_cast<unsigned*>(abort_flag) + kFlagStride;
+  hipError_t err = hipMemsetAsync(abort_flag, 0, size_t(kFlagStride) * sizeof(unsigned) * (1 + size_t(groups)), s);
   if (err != hipSuccess) return err;
   const Z* az = reinterpret_cast<const Z*>(a);
   Z* rz = reinterpret_cast<Z*>(refl);
   Z* tz = reinterpret_cast<Z*>(tau);
   void* args[] = {(void*)&az, (void*)&lda, (void*)&n, (void*)&rpw, (void*)&rz, (void*)&ldr, (void*)&d,
-                  (void*)&e, (void*)&tz, (void*)&pub_p, (void*)&pub_c, (void*)&counter, (void*)&abort_flag};
+                  (void*)&e, (void*)&tz, (void*)&pub_p, (void*)&pub_c, (void*)&flags, (void*)&abort_flag};
   return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_hetrd_lds), dim3(groups), dim3(kThreads), args,
                                     static_cast<unsigned>(lds_bytes(n, rpw)), s);
 }
 
 const int* hetrd_abort_flag(const void* workspace, int n) {
   const Z* pub = reinterpret_cast<const Z*>(workspace);
-  return reinterpret_cast<const int*>(reinterpret_cast<const unsigned*>(pub + 4 * size_t(n)) + 1);
+  return reinterpret_cast<const int*>(pub + 4 * size_t(n));
 }
 
 }  // namespace sdft_hip

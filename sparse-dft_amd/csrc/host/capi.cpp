@@ -549,6 +549,44 @@ int sdft_hip_operator_projector_metric(sdft_hip_operator* op, const double* p, s
   });
 }
 
+int sdft_hip_operator_real_space_search(sdft_hip_operator* op, double factor, const double* lattice,
+                                        const double* fractional, const double* radius, size_t* points,
+                                        double* retained) {
+  return guarded([&] {
+    if (!op) throw std::invalid_argument("HIP: null operator");
+    const long kept = op->impl.enable_real_space_search(factor, lattice, fractional, radius, retained);
+    if (points) *points = size_t(kept);
+  });
+}
+
+int sdft_hip_operator_apply_real_space(sdft_hip_operator* op, int which, const double* x, size_t columns, double* y,
+                                       int precision) {
+  return guarded([&] {
+    if (!op || !x || !y || !columns || (which != 0 && which != 1) || (precision != 0 && precision != 1))
+      throw std::invalid_argument("HIP: invalid real-space search arguments");
+    auto& o = op->impl;
+    hipStream_t s = o.context().stream();
+    const size_t n = o.n();
+    sdft_hip::DevBuf<double> dx, dy;
+    dx.upload(x, 2 * n * columns, s);
+    dy.reserve(2 * n * columns);
+    o.apply_real_space(which, dx.p, long(columns), dy.p,
+                       precision ? sdft_hip::Precision::FP32 : sdft_hip::Precision::FP64);
+    sdft_hip::check(hipMemcpyAsync(y, dy.p, 16 * n * columns, hipMemcpyDeviceToHost, s), "download");
+    o.context().sync();
+  });
+}
+
+int sdft_hip_operator_real_space_points(sdft_hip_operator* op, long* natural, long* per_atom) {
+  return guarded([&] {
+    if (!op) throw std::invalid_argument("HIP: null operator");
+    std::vector<long> nat, counts;
+    op->impl.real_space_points(nat, counts);
+    if (natural) std::copy(nat.begin(), nat.end(), natural);
+    if (per_atom) std::copy(counts.begin(), counts.end(), per_atom);
+  });
+}
+
 int sdft_hip_operator_density(sdft_hip_operator* op, const double* coefficients, size_t bands,
                               const double* occupations, double volume, size_t nsupport, const int* support_hkl,
                               double* out) {

@@ -11,6 +11,7 @@
 #include <sstream>
 #include <tuple>
 
+#include "../hip/slot_order.hpp"
 #include "../hip/fft_lengths.hpp"
 #include "snapshot.hpp"
 
@@ -550,10 +551,226 @@ void DeviceOperator::apply(int which, const double* x, long columns, double* hy,
   if (columns <= 0 || !x) throw std::invalid_argument("HIP: invalid plane-wave block operator buffers");
   if ((which != 1 && (!hy || hy == x)) || (which != 0 && (!sy || sy == x)) || (which == 2 && hy == sy))
     throw std::invalid_argument("HIP: invalid direct plane-wave input/output buffers");
+  if (rs_enabled_ && prec == Precision::FP32) {
+    // search applications through the real-space projectors (src/plane_wave_operator.cpp:221-223)
+    if (which != 1) apply_real_space(0, x, columns, hy, prec);
+    if (which != 0) apply_real_space(1, x, columns, sy, prec);
+    return;
+  }
   for (long c0 = 0; c0 < columns; c0 += long(chunk_)) {
     const long c = std::min<long>(long(chunk_), columns - c0);
     const long off = 2 * c0 * long(n_);
     apply_chunk(which, x + off, c, hy ? hy + off : nullptr, sy ? sy + off : nullptr, prec);
+  }
+}
+
+long DeviceOperator::enable_real_space_search(double factor, const double* lattice, const double* fractional,
+                                              const double* radius, double* retained) {
+  using C = std::complex<double>;
+  if (factor == 0) {
+    rs_enabled_ = false;
+    return 0;
+  }
+  if (!std::isfinite(factor) || factor < 1. || factor > 2.)
+    throw std::invalid_argument("SDFT_PROJECTOR_RADIUS_FACTOR must be in [1,2]");
+  if (!np_ || form_ != 3) throw std::invalid_argument("HIP: real-space search needs atom-block PAW projectors");
+  const double threshold = std::pow(10., -2. - 5. * (factor - 1.));
+  hipStream_t s = ctx_.stream();
+  const long n = long(n_), count = count_;
+  std::vector<C> b(np_ * n_);
+  check(hipMemcpyAsync(b.data(), b64_.p, 16 * b.size(), hipMemcpyDeviceToHost, s), "projector copy");
+  ctx_.sync();
+  int rmax = 0;
+  for (int r : atom_rows_host_) rmax = std::max(rmax, r);
+  DevBuf<double> xin, peak, w64a;
+  DevBuf<float> w32a;
+  DevBuf<char> t1, planes;
+  DevBuf<int> pts;
+  xin.reserve(std::size_t(2 * n * rmax));
+  peak.reserve(std::size_t(count));
+  t1.reserve(std::size_t(rmax) * grid_.dev.nsticks * shape_[2] * 16);
+  planes.reserve(std::size_t(rmax) * count * 16);
+  std::vector<long> woff, poff{0};
+  std::vector<int> aoff, all_points;
+  std::vector<C> w_all;
+  std::vector<std::array<int, 3>> entries;  // (slot point, atom, local index)
+  std::vector<C> xa;
+  std::vector<double> pk(count);
+  long row0 = 0;
+  double kept_power = 0, total_power = 0;
+  const int n0 = shape_[0], n1 = shape_[1], n2 = shape_[2];
+  const int a0 = grid_.dev.split[0], a1 = grid_.dev.split[1], a2 = grid_.dev.split[2];
+  // natural grid coordinates of slot-order plane index i ([zs][xs][ys])
+  auto natural_xyz = [&](long i, int& x, int& y, int& z) {
+    const int ys = int(i % n1), xs = int((i / n1) % n0), zs = int(i / (long(n0) * n1));
+    x = slot_position(xs, a0, n0 / a0);
+    y = slot_position(ys, a1, n1 / a1);
+    z = slot_position(zs, a2, n2 / a2);
+  };
+  for (std::size_t a = 0; a < atom_rows_host_.size(); ++a) {
+    const int rows = atom_rows_host_[a];
+    // w_i(r) = FFT_fwd(B_i)(r) / N = conj(backward(conj B_i))(r) / N
+    xa.assign(std::size_t(n) * rows, C{});
+    for (int i = 0; i < rows; ++i)
+      for (long g = 0; g < n; ++g) xa[std::size_t(i) * n + g] = std::conj(b[std::size_t(row0 + i) + std::size_t(g) * np_]);
+    check(hipMemcpyAsync(xin.p, xa.data(), 16 * xa.size(), hipMemcpyHostToDevice, s), "projector rows");
+    check(launch_backward_full<double>(grid_.dev, xin.p, n, rows, t1.p, planes.p, s), "projector transforms");
+    check(launch_rs_peak(planes.p, count, rows, peak.p, s), "projector peak");
+    check(hipMemcpyAsync(pk.data(), peak.p, 8 * std::size_t(count), hipMemcpyDeviceToHost, s), "peak copy");
+    ctx_.sync();
+    // peak holds N^2 sum_i |w_i|^2: keep |w(r)| > threshold max |w|, or the
+    // points of the atom's sphere of radius factor * radius
+    const double top = *std::max_element(pk.begin(), pk.end());
+    const bool sphere = lattice && fractional && radius && radius[a] > 0;
+    const double cut = sphere ? factor * radius[a] : 0.;
+    std::vector<int> keep;
+    double kept_a = 0, total_a = 0;
+    for (long r = 0; r < count; ++r) {
+      total_a += pk[r];
+      bool k;
+      if (sphere) {
+        int ix, iy, iz;
+        natural_xyz(r, ix, iy, iz);
+        double f[3] = {double(ix) / n0 - fractional[3 * a], double(iy) / n1 - fractional[3 * a + 1],
+                       double(iz) / n2 - fractional[3 * a + 2]};
+        for (double& v : f) v -= std::round(v);
+        double d2 = 0;
+        for (int c = 0; c < 3; ++c) {
+          const double u = f[0] * lattice[c] + f[1] * lattice[3 + c] + f[2] * lattice[6 + c];
+          d2 += u * u;
+        }
+        k = d2 <= cut * cut;
+      } else {
+        k = pk[r] > threshold * threshold * top;
+      }
+      if (k) {
+        keep.push_back(int(r));
+        kept_a += pk[r];
+      }
+    }
+    kept_power += kept_a;
+    total_power += total_a;
+    const long np_a = long(keep.size());
+    pts.upload(keep.data(), keep.size(), s);
+    w64a.reserve(std::size_t(2 * rows * std::max<long>(np_a, 1)));
+    w32a.reserve(std::size_t(2 * rows * std::max<long>(np_a, 1)));
+    check(launch_rs_values(planes.p, count, rows, pts.p, np_a, w64a.p, w32a.p, s), "projector values");
+    std::vector<C> wa(std::size_t(rows) * np_a);
+    check(hipMemcpyAsync(wa.data(), w64a.p, 16 * wa.size(), hipMemcpyDeviceToHost, s), "values copy");
+    ctx_.sync();
+    woff.push_back(long(w_all.size()));
+    w_all.insert(w_all.end(), wa.begin(), wa.end());
+    for (long j = 0; j < np_a; ++j) entries.push_back({keep[j], int(a), int(j)});
+    all_points.insert(all_points.end(), keep.begin(), keep.end());
+    poff.push_back(poff.back() + np_a);
+    aoff.push_back(int(row0));
+    row0 += rows;
+  }
+  // point-major CSR of (atom, local index), atoms ascending within a point
+  std::sort(entries.begin(), entries.end());
+  std::vector<int> upoints, estart, eatom, elocal;
+  for (std::size_t t = 0; t < entries.size(); ++t) {
+    if (t == 0 || entries[t][0] != entries[t - 1][0]) {
+      upoints.push_back(entries[t][0]);
+      estart.push_back(int(t));
+    }
+    eatom.push_back(entries[t][1]);
+    elocal.push_back(entries[t][2]);
+  }
+  estart.push_back(int(entries.size()));
+  rs_w64_.upload(reinterpret_cast<const double*>(w_all.data()), 2 * w_all.size(), s);
+  std::vector<float> w32(2 * w_all.size());
+  for (std::size_t i = 0; i < w32.size(); ++i) w32[i] = float(reinterpret_cast<const double*>(w_all.data())[i]);
+  rs_w32_.upload(w32.data(), w32.size(), s);
+  rs_woff_.upload(woff.data(), woff.size(), s);
+  rs_poff_.upload(poff.data(), poff.size(), s);
+  rs_points_.upload(all_points.data(), all_points.size(), s);
+  rs_upoints_.upload(upoints.data(), upoints.size(), s);
+  rs_estart_.upload(estart.data(), estart.size(), s);
+  rs_eatom_.upload(eatom.data(), eatom.size(), s);
+  rs_elocal_.upload(elocal.data(), elocal.size(), s);
+  rs_aoff_.upload(aoff.data(), aoff.size(), s);
+  std::vector<double> sd(n_, 1.0 + reg_);
+  rs_sdiag_.upload(sd.data(), sd.size(), s);
+  ctx_.sync();
+  rs_ = RealSpaceDev{};
+  rs_.atoms = int(atom_rows_host_.size());
+  rs_.total_points = long(all_points.size());
+  rs_.unique = long(upoints.size());
+  rs_.w64 = rs_w64_.p;
+  rs_.w32 = rs_w32_.p;
+  rs_.w_offset = rs_woff_.p;
+  rs_.point_offset = rs_poff_.p;
+  rs_.atom_offset = rs_aoff_.p;
+  rs_.atom_rows = atom_rows_.p;
+  rs_.points = rs_points_.p;
+  rs_.unique_points = rs_upoints_.p;
+  rs_.entry_start = rs_estart_.p;
+  rs_.entry_atom = rs_eatom_.p;
+  rs_.entry_local = rs_elocal_.p;
+  // natural indices of the retained points, for export
+  rs_natural_.clear();
+  rs_counts_.clear();
+  for (std::size_t a = 0; a + 1 < poff.size(); ++a) {
+    rs_counts_.push_back(poff[a + 1] - poff[a]);
+    for (long j = poff[a]; j < poff[a + 1]; ++j) {
+      int x, y, z;
+      natural_xyz(all_points[j], x, y, z);
+      rs_natural_.push_back((long(x) * n1 + y) * n2 + z);
+    }
+  }
+  if (retained) *retained = total_power > 0 ? kept_power / total_power : 1.;
+  rs_enabled_ = true;
+  return rs_.total_points;
+}
+
+void DeviceOperator::real_space_points(std::vector<long>& natural, std::vector<long>& per_atom) const {
+  natural = rs_natural_;
+  per_atom = rs_counts_;
+}
+
+// src/plane_wave_operator.cpp:169-212 with the FFTs on the device
+void DeviceOperator::apply_real_space(int which, const double* x, long columns, double* y, Precision prec) {
+  if (!rs_enabled_) throw std::invalid_argument("HIP: real-space search is not enabled");
+  if ((which != 0 && which != 1) || columns <= 0 || !x || !y || x == y)
+    throw std::invalid_argument("HIP: invalid real-space search buffers");
+  hipStream_t s = ctx_.stream();
+  const long n = long(n_), np = long(np_), count = count_;
+  const bool f64 = prec == Precision::FP64;
+  const std::size_t csz = f64 ? 16 : 8;
+  const long chunk = std::max<long>(1, std::min<long>(columns, (256L << 20) / (count * long(csz))));
+  rs_planes_.reserve(std::size_t(chunk) * count * csz);
+  rs_psi_.reserve(std::size_t(std::max<long>(rs_.total_points, 1)) * chunk * csz);
+  t1_.reserve(std::size_t(chunk) * grid_.dev.nsticks * shape_[2] * csz);
+  p_.reserve(std::size_t(2 * np * chunk));
+  w_.reserve(std::size_t(2 * np * 2 * chunk));
+  const double* diag = which ? rs_sdiag_.p : kinetic_d_.p;
+  for (long c0 = 0; c0 < columns; c0 += chunk) {
+    const long c = std::min(chunk, columns - c0);
+    const double* xc = x + 2 * c0 * n;
+    double* yc = y + 2 * c0 * n;
+    if (f64) {
+      check(launch_backward_full<double>(grid_.dev, xc, n, c, t1_.p, rs_planes_.p, s), "real-space backward");
+      check(launch_rs_gather<double>(rs_planes_.p, count, rs_.points, rs_.total_points, int(c), rs_psi_.p, s), "gather");
+      check(launch_rs_project<double>(rs_, rs_psi_.p, int(c), p_.p, np, s), "projection");
+    } else {
+      check(launch_backward_full<float>(grid_.dev, xc, n, c, t1_.p, rs_planes_.p, s), "real-space backward");
+      check(launch_rs_gather<float>(rs_planes_.p, count, rs_.points, rs_.total_points, int(c), rs_psi_.p, s), "gather");
+      check(launch_rs_project<float>(rs_, rs_psi_.p, int(c), p_.p, np, s), "projection");
+    }
+    check(launch_atom_blocks(prec, int(np), int(c), atoms_, atom_offset_.p, atom_rows_.p, block_offset_.p, dblk_.p,
+                             qblk_.p, p_.p, np, w_.p, np, which ? 2 : 1, s),
+          "onsite blocks");
+    const double* amp = w_.p + (which ? 2 * np * c : 0);
+    if (f64) {
+      check(launch_rs_images<double>(rs_, rs_planes_.p, count, int(c), v64_.p, which, amp, np, s), "images");
+      check(launch_forward_planes<double>(grid_.dev, rs_planes_.p, c, xc, n, diag, yc, n, t2_.reserve(std::size_t(c) * grid_.dev.nsticks * shape_[2] * csz), s),
+            "real-space forward");
+    } else {
+      check(launch_rs_images<float>(rs_, rs_planes_.p, count, int(c), v32_.p, which, amp, np, s), "images");
+      check(launch_forward_planes<float>(grid_.dev, rs_planes_.p, c, xc, n, diag, yc, n, t2_.reserve(std::size_t(c) * grid_.dev.nsticks * shape_[2] * csz), s),
+            "real-space forward");
+    }
   }
 }
 

@@ -16,6 +16,7 @@
 
 #include "../hip/gemm.hpp"
 #include "../hip/local_h.hpp"
+#include "../hip/realspace.hpp"
 
 namespace sdft_hip {
 
@@ -193,6 +194,21 @@ class DeviceOperator {
   std::vector<std::complex<double>> density(const double* x, long bands, const double* occupations, double volume,
                                             const int* support, long nsupport);
   void projector_metric(const double* p, long columns, double* qp);
+  // Real-space projector search (src/plane_wave_operator.cpp:104-212). Per
+  // atom, an atom with a radius > 0 (geometry given) keeps the grid points
+  // within factor * radius of its centre (minimum image); otherwise the points
+  // where the projector amplitude exceeds 10^(-2 - 5 (factor - 1)) of its
+  // maximum. factor 0 disables. While enabled, FP32 (search) applications use
+  // it. Returns the retained points; *retained gets the kept fraction of
+  // sum_r sum_i |w_i(r)|^2. lattice: 3 rows of cell vectors (bohr);
+  // fractional: atoms x 3; radius: atoms (any of them may be null).
+  long enable_real_space_search(double factor, const double* lattice = nullptr, const double* fractional = nullptr,
+                                const double* radius = nullptr, double* retained = nullptr);
+  bool real_space_search() const { return rs_enabled_; }
+  // which 0: H, 1: S through the real-space projectors, device pointers
+  void apply_real_space(int which, const double* x, long columns, double* y, Precision prec);
+  // retained points per atom (natural grid indices, atom by atom)
+  void real_space_points(std::vector<long>& natural, std::vector<long>& per_atom) const;
   std::size_t n() const { return n_; }
   std::size_t np() const { return np_; }
   long count() const { return count_; }
@@ -231,6 +247,14 @@ class DeviceOperator {
   int atoms_ = 0;
   DevBuf<char> t1_, t2_;
   DevBuf<double> p_, w_;
+  bool rs_enabled_ = false;
+  RealSpaceDev rs_{};
+  DevBuf<double> rs_w64_, rs_sdiag_;
+  DevBuf<float> rs_w32_;
+  DevBuf<long> rs_woff_, rs_poff_;
+  DevBuf<int> rs_points_, rs_upoints_, rs_estart_, rs_eatom_, rs_elocal_, rs_aoff_;
+  DevBuf<char> rs_planes_, rs_psi_;
+  std::vector<long> rs_natural_, rs_counts_;
 };
 
 // reference index map (src/plane_wave_operator.cpp:94-97)

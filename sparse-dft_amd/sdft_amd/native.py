@@ -321,6 +321,39 @@ class Operator:
         _check(lib().sdft_hip_operator_projector_metric(self.handle, _p(p), ct.c_size_t(p.shape[1]), _p(qp)))
         return qp
 
+    def real_space_search(self, factor=1.3, lattice=None, fractional=None, radius=None):
+        """Enable (factor in [1, 2]) or disable (0) the real-space projector
+        search for FP32 search applications. With geometry (lattice rows in
+        bohr, fractional atoms x 3, radius per atom) atoms of radius > 0 keep
+        the points of their sphere; otherwise the amplitude threshold is used.
+        Returns (retained points, retained projector power fraction)."""
+        kept = ct.c_size_t()
+        power = ct.c_double()
+        geo = [None, None, None]
+        if lattice is not None:
+            geo = [np.ascontiguousarray(lattice, dtype=np.float64).reshape(9),
+                   np.ascontiguousarray(fractional, dtype=np.float64).reshape(-1),
+                   np.ascontiguousarray(radius, dtype=np.float64).reshape(-1)]
+        _check(lib().sdft_hip_operator_real_space_search(
+            self.handle, ct.c_double(factor), *[None if g is None else _p(g) for g in geo], ct.byref(kept),
+            ct.byref(power)))
+        return kept.value, power.value
+
+    def apply_real_space(self, x, which, precision=0):
+        x = np.asfortranarray(x, dtype=np.complex128)
+        y = np.zeros_like(x, order="F")
+        _check(lib().sdft_hip_operator_apply_real_space(self.handle, ct.c_int({"h": 0, "s": 1}[which]), _p(x),
+                                                        ct.c_size_t(x.shape[1]), _p(y), ct.c_int(precision)))
+        return y
+
+    def real_space_points(self, atoms):
+        """(natural grid indices of the retained points, atom by atom; count per atom)."""
+        counts = np.zeros(atoms, dtype=np.int64)
+        _check(lib().sdft_hip_operator_real_space_points(self.handle, None, _p(counts, _lp)))
+        natural = np.zeros(int(counts.sum()), dtype=np.int64)
+        _check(lib().sdft_hip_operator_real_space_points(self.handle, _p(natural, _lp), _p(counts, _lp)))
+        return natural, counts
+
     def density(self, coefficients, occupations, volume, support_hkl):
         """Smooth density Fourier coefficients at support_hkl (m x 3 ints),
         src/density_functionals.cpp:406-460."""
