@@ -50,8 +50,14 @@ __device__ __forceinline__ void load2(const S* p, double& re, double& im) {
 // ones, so that no MFMA waits on the accumulator of the one just issued.
 // PF: the next K slab is loaded into registers while the current one is
 // multiplied from LDS.
+// At least SDFT_GEMM_MIN_WAVES waves per SIMD: left alone, the FP64 tiles keep
+// their accumulators in AGPRs and the alpha/beta epilogue copies them all to
+// VGPRs at once, which caps occupancy at two waves.
+#ifndef SDFT_GEMM_MIN_WAVES
+#define SDFT_GEMM_MIN_WAVES 3
+#endif
 template <typename T, typename TA, typename TB, bool ADJ_A, bool ADJ_B, int TBM, int TBN, int WGM, bool PF>
-__global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, const TA* __restrict__ A, long lda,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDFT_GEMM_MIN_WAVES))) void k_zgemm(int M, int N, int K, const TA* __restrict__ A, long lda,
                                                const TB* __restrict__ B, long ldb, double* __restrict__ C, long ldc,
                                                const double* __restrict__ D, long ldd, double alpha, double beta,
                                                int kps, double* __restrict__ work) {

@@ -16,6 +16,8 @@
 // buffers (nsticks * N2 values each) plus psi in/out.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "fft_device.hpp"
 #include "fft_lengths.hpp"
 #include "local_h.hpp"
@@ -81,9 +83,8 @@ __global__ __launch_bounds__(256) void k_sticks_backward(const double* __restric
 // mode 1: backward only, whole plane stored in slot order (potential setup).
 // In mode 0 the y-transforms run only on the x-lines that hold sticks (the
 // backward input and the forward output are zero / unused elsewhere).
-constexpr int kPlaneThreads = 512;
-template <typename T>
-__global__ __launch_bounds__(kPlaneThreads) void k_planes(const cx<T>* __restrict__ t1, cx<T>* __restrict__ t2,
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_planes(const cx<T>* __restrict__ t1, cx<T>* __restrict__ t2,
                                                           const int* __restrict__ stick_xy, int nsticks, int n0,
                                                           int n1, int n2, const cx<T>* __restrict__ tw_x,
                                                           const cx<T>* __restrict__ tw_y,
@@ -131,6 +132,31 @@ __global__ __launch_bounds__(kPlaneThreads) void k_planes(const cx<T>* __restric
   if (xl1) line_fft_dispatch<T, -1, false>(n1, plane + xs1 * n1, gy1, tw_y, tid, nt);
   cx<T>* out = t2 + (col * n2 + zs) * (long)nsticks;
   for (int s = tid; s < nsticks; s += nt) out[s] = plane[stick_xy[s]];
+}
+
+// Threads per plane workgroup (one xy-plane per workgroup): 512 unless
+// SDFT_HIP_PLANE_THREADS selects 256 or 1024.
+int plane_threads() {
+  static const int threads = [] {
+    const char* e = std::getenv("SDFT_HIP_PLANE_THREADS");
+    const int t = e ? std::atoi(e) : 512;
+    return t == 256 || t == 1024 ? t : 512;
+  }();
+  return threads;
+}
+
+template <typename T, typename... Args>
+void launch_k_planes(dim3 grid, size_t lds, hipStream_t stream, Args... args) {
+  switch (plane_threads()) {
+    case 256:
+      hipLaunchKernelGGL((k_planes<T, 256>), grid, dim3(256), lds, stream, args...);
+      break;
+    case 1024:
+      hipLaunchKernelGGL((k_planes<T, 1024>), grid, dim3(1024), lds, stream, args...);
+      break;
+    default:
+      hipLaunchKernelGGL((k_planes<T, 512>), grid, dim3(512), lds, stream, args...);
+  }
 }
 
 // C: grid (ceil(nsticks / spb), columns). y = T x + local / N
@@ -268,7 +294,7 @@ hipError_t launch_local_h(const GridPlanDev& p, const double* x, long ldx, long 
   hipLaunchKernelGGL(k_sticks_backward<T>, ga, dim3(256), lds_lines, stream, x, ldx, p.stick_g, p.nsticks, p.n2, spb,
                      twz, reinterpret_cast<cx<T>*>(t1));
   const size_t lds_plane = size_t(p.n0) * p.n1 * sizeof(cx<T>);
-  hipLaunchKernelGGL(k_planes<T>, dim3(p.n2, (unsigned)columns), dim3(kPlaneThreads), lds_plane, stream,
+  launch_k_planes<T>(dim3(p.n2, (unsigned)columns), lds_plane, stream,
                      reinterpret_cast<const cx<T>*>(t1), reinterpret_cast<cx<T>*>(t2), p.stick_xy, p.nsticks, p.n0,
                      p.n1, p.n2, twx, twy, v, 0, p.xstart[0], p.xlines[0], p.xstart[1], p.xlines[1]);
   hipLaunchKernelGGL(k_sticks_forward<T>, ga, dim3(256), lds_lines, stream, reinterpret_cast<const cx<T>*>(t2),
@@ -299,7 +325,7 @@ hipError_t launch_potential_setup(const GridPlanDev& full, const double* grid, v
   hipLaunchKernelGGL(k_sticks_backward<double>, ga, dim3(256), lds_lines, stream, grid, 0L, full.stick_g, full.nsticks,
                      full.n2, spb, reinterpret_cast<const cx<double>*>(full.tw64[2]), reinterpret_cast<cx<double>*>(t1));
   const size_t lds_plane = size_t(full.n0) * full.n1 * sizeof(cx<double>);
-  hipLaunchKernelGGL(k_planes<double>, dim3(full.n2, 1), dim3(kPlaneThreads), lds_plane, stream,
+  launch_k_planes<double>(dim3(full.n2, 1), lds_plane, stream,
                      reinterpret_cast<const cx<double>*>(t1), reinterpret_cast<cx<double>*>(t2), full.stick_xy,
                      full.nsticks, full.n0, full.n1, full.n2, reinterpret_cast<const cx<double>*>(full.tw64[0]),
                      reinterpret_cast<const cx<double>*>(full.tw64[1]), (const double*)nullptr, 1, 0, full.n0, 0, 0);
@@ -369,7 +395,7 @@ hipError_t launch_backward_planes(const GridPlanDev& p, const double* x, long ld
   hipLaunchKernelGGL(k_sticks_backward<double>, ga, dim3(256), lds_lines, stream, x, ldx, p.stick_g, p.nsticks, p.n2,
                      spb, reinterpret_cast<const cx<double>*>(p.tw64[2]), reinterpret_cast<cx<double>*>(t1));
   const size_t lds_plane = size_t(p.n0) * p.n1 * sizeof(cx<double>);
-  hipLaunchKernelGGL(k_planes<double>, dim3(p.n2, (unsigned)columns), dim3(kPlaneThreads), lds_plane, stream,
+  launch_k_planes<double>(dim3(p.n2, (unsigned)columns), lds_plane, stream,
                      reinterpret_cast<const cx<double>*>(t1), reinterpret_cast<cx<double>*>(t2), p.stick_xy,
                      p.nsticks, p.n0, p.n1, p.n2, reinterpret_cast<const cx<double>*>(p.tw64[0]),
                      reinterpret_cast<const cx<double>*>(p.tw64[1]), (const double*)nullptr, 1, p.xstart[0],
@@ -391,7 +417,7 @@ hipError_t launch_backward_full(const GridPlanDev& p, const double* x, long ldx,
   hipLaunchKernelGGL(k_sticks_backward<T>, ga, dim3(256), lds_lines, stream, x, ldx, p.stick_g, p.nsticks, p.n2, spb,
                      twz, reinterpret_cast<cx<T>*>(t1));
   const size_t lds_plane = size_t(p.n0) * p.n1 * sizeof(cx<T>);
-  hipLaunchKernelGGL(k_planes<T>, dim3(p.n2, (unsigned)columns), dim3(kPlaneThreads), lds_plane, stream,
+  launch_k_planes<T>(dim3(p.n2, (unsigned)columns), lds_plane, stream,
                      reinterpret_cast<const cx<T>*>(t1), reinterpret_cast<cx<T>*>(t2), p.stick_xy, p.nsticks, p.n0,
                      p.n1, p.n2, twx, twy, (const T*)nullptr, 1, p.xstart[0], p.xlines[0], p.xstart[1], p.xlines[1]);
   return hipGetLastError();
@@ -406,7 +432,7 @@ hipError_t launch_forward_planes(const GridPlanDev& p, const void* planes, long 
   const cx<T>* twy = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[1] : p.tw32[1]);
   const cx<T>* twz = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[2] : p.tw32[2]);
   const size_t lds_plane = size_t(p.n0) * p.n1 * sizeof(cx<T>);
-  hipLaunchKernelGGL(k_planes<T>, dim3(p.n2, (unsigned)columns), dim3(kPlaneThreads), lds_plane, stream,
+  launch_k_planes<T>(dim3(p.n2, (unsigned)columns), lds_plane, stream,
                      reinterpret_cast<const cx<T>*>(planes), reinterpret_cast<cx<T>*>(t_sticks), p.stick_xy,
                      p.nsticks, p.n0, p.n1, p.n2, twx, twy, (const T*)nullptr, 2, p.xstart[0], p.xlines[0],
                      p.xstart[1], p.xlines[1]);

@@ -18,6 +18,11 @@
 // per workgroup, no read-modify-write contention) and polls all flags with one
 // wave, with a bounded wait: on expiry it raises an abort flag that every
 // workgroup observes, and all workgroups leave the kernel.
+// Every published value is stored and loaded with agent-scope relaxed atomics
+// (sc1 accesses, coherent across the XCDs' L2 caches); a workgroup raises its
+// flag only after its stores have completed (vmcnt(0)), and readers load the
+// values only after they have seen every flag. This needs no L2 write-back or
+// invalidate, which an agent-scope fence would issue on every column.
 #include <hip/hip_runtime.h>
 
 #include "tridiag.hpp"
@@ -28,6 +33,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr long kSpinLimit = 1L << 20;
+constexpr int kPerThread = (kHetrdMaxOrder + kThreads - 1) / kThreads;  // trailing entries per thread
 
 struct Z {
   double x, y;
@@ -76,49 +82,56 @@ __device__ Z block_zsum(Z v, double* scratch) {
   return t;
 }
 
-// Grid barrier with publication: thread 0 writes the workgroup's staged
-// values (count entries of `src`, LDS, to dst[index[j]]), releases them at agent
-// scope and raises its own arrival flag to `step` (one flag per workgroup, each
-// on its own 64-byte line, so arrivals never contend); the lanes of wave 0 then
-// poll every workgroup's flag in parallel and acquire. Returns false when the
-// wait expired or another workgroup gave up.
+__device__ inline void store_published(Z* p, Z v) {
+  double* q = reinterpret_cast<double*>(p);
+  __hip_atomic_store(q, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline Z load_published(const Z* p) {
+  const double* q = reinterpret_cast<const double*>(p);
+  return {__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+          __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+}
+// s_waitcnt vmcnt(0) with the other counters unconstrained (gfx9 encoding)
+constexpr int kWaitStores = 0x0F70;
+
+// Grid barrier with publication: the workgroup's staged values (count entries
+// of `src`, LDS, to dst[index[j]]) are stored by as many threads, each waits
+// for its stores to complete, and thread 0 raises the workgroup's arrival flag
+// to `step` (one flag per workgroup, each on its own 64-byte line, so arrivals
+// never contend). Thread q then polls the flag of workgroup q (every flag is
+// watched at once, so the wait ends one memory round trip after the last
+// arrival). Returns false when the wait expired or another workgroup gave up.
 constexpr int kFlagStride = 16;  // unsigned ints per flag line
-__device__ bool publish_and_wait(unsigned* flags, int* abort_flag, unsigned step, int* ok_shared, int count,
-                                 const int* index, const Z* src_p, const Z* src_c, Z* dst_p, Z* dst_c) {
+__device__ bool publish_and_wait(unsigned* flags, int* abort_flag, unsigned step, int count, const int* index,
+                                 const Z* src_p, const Z* src_c, Z* dst_p, Z* dst_c) {
   __syncthreads();
   const int tid = threadIdx.x;
-  if (tid == 0) {
-    for (int j = 0; j < count; ++j) {
-      dst_p[index[j]] = src_p[j];
-      dst_c[index[j]] = src_c[j];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_store(flags + blockIdx.x * kFlagStride, step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int j = tid; j < count; j += kThreads) {
+    store_published(dst_p + index[j], src_p[j]);
+    store_published(dst_c + index[j], src_c[j]);
   }
-  if (tid < 64) {
-    int ok = 1;
-    for (int q = tid; q < int(gridDim.x) && ok; q += 64) {
-      for (long spin = 0;; ++spin) {
-        if (__hip_atomic_load(flags + q * kFlagStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= step) break;
-        if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-          ok = 0;
-          break;
-        }
-        if (spin > kSpinLimit) {
-          __hip_atomic_fetch_or(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          ok = 0;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    // every lane of wave 0 must have seen its flags
-    const unsigned long long all_ok = __ballot(ok);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (tid == 0) *ok_shared = all_ok == ~0ull ? 1 : 0;
-  }
+  __builtin_amdgcn_s_waitcnt(kWaitStores);
   __syncthreads();
-  return *ok_shared != 0;
+  if (tid == 0)
+    __hip_atomic_store(flags + blockIdx.x * kFlagStride, step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int ok = 1;
+  for (int q = tid; q < int(gridDim.x) && ok; q += kThreads) {
+    for (long spin = 0;; ++spin) {
+      if (__hip_atomic_load(flags + q * kFlagStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= step) break;
+      if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        ok = 0;
+        break;
+      }
+      if (spin > kSpinLimit) {
+        __hip_atomic_fetch_or(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  return __syncthreads_and(ok) != 0;
 }
 
 __global__ __launch_bounds__(kThreads) void k_hetrd_lds(const Z* __restrict__ a, long lda, int n, int rpw,
@@ -135,7 +148,7 @@ __global__ __launch_bounds__(kThreads) void k_hetrd_lds(const Z* __restrict__ a,
   Z* stage_p = reinterpret_cast<Z*>(scratch + 2 * kWaves);  // rpw staged p_i
   Z* stage_c = stage_p + rpw;                               // rpw staged A_{i,k+1}
   int* stage_i = reinterpret_cast<int*>(stage_c + rpw);     // rpw row indices
-  __shared__ int ok_shared, staged;
+  __shared__ int staged;
   __shared__ Z sh_a2;
 
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
@@ -207,13 +220,27 @@ __global__ __launch_bounds__(kThreads) void k_hetrd_lds(const Z* __restrict__ a,
     }
     // ---- 3. publication and grid barrier
     __syncthreads();
-    if (!publish_and_wait(flags, abort_flag, unsigned(k + 1), &ok_shared, staged, stage_i, stage_p,
-                          stage_c, pp, pc))
-      return;
-    // ---- 4. w, rank-2 update of own rows, next column for all rows
-    for (int j = tid; j < m; j += kThreads) {
-      w[j] = pp[k + 1 + j];
-      col[k + 1 + j] = pc[k + 1 + j];
+    if (!publish_and_wait(flags, abort_flag, unsigned(k + 1), staged, stage_i, stage_p, stage_c, pp, pc)) return;
+    // ---- 4. w, rank-2 update of own rows, next column for all rows; all
+    // loads are issued before the first is used (one round trip)
+    {
+      Z pw[kPerThread], pcv[kPerThread];
+#pragma unroll
+      for (int q = 0; q < kPerThread; ++q) {
+        const int j = tid + q * kThreads;
+        if (j < m) {
+          pw[q] = load_published(pp + k + 1 + j);
+          pcv[q] = load_published(pc + k + 1 + j);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kPerThread; ++q) {
+        const int j = tid + q * kThreads;
+        if (j < m) {
+          w[j] = pw[q];
+          col[k + 1 + j] = pcv[q];
+        }
+      }
     }
     __syncthreads();
     Z dot{0., 0.};  // (tau p)^H v

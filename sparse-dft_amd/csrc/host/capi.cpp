@@ -460,16 +460,21 @@ int sdft_hip_operator_benchmark(sdft_hip_operator* op, int which, size_t columns
     // local-H pipeline alone (which = 0 without projectors is not separable:
     // time the S-free path of H minus the projector chain via a local-only run)
     check(hipEventRecord(e0, s), "event record");
-    for (int r = 0; r < repetitions; ++r) {
-      if (prec == Precision::FP64)
-        check(sdft_hip::launch_local_h<double>(o.grid_plan(), dx.p, long(o.n()), long(columns), o.kinetic_device(), dh.p,
-                                               long(o.n()), o.stick_buffer(0, columns, prec), o.stick_buffer(1, columns, prec), s),
-              "local H");
-      else
-        check(sdft_hip::launch_local_h<float>(o.grid_plan(), dx.p, long(o.n()), long(columns), o.kinetic_device(), dh.p,
-                                              long(o.n()), o.stick_buffer(0, columns, prec), o.stick_buffer(1, columns, prec), s),
-              "local H");
-    }
+    const long chunk = long(o.max_chunk()), n = long(o.n());
+    for (int r = 0; r < repetitions; ++r)
+      for (long c0 = 0; c0 < long(columns); c0 += chunk) {
+        const long c = std::min<long>(chunk, long(columns) - c0);
+        void* t1 = o.stick_buffer(0, std::size_t(c), prec);
+        void* t2 = o.stick_buffer(1, std::size_t(c), prec);
+        if (prec == Precision::FP64)
+          check(sdft_hip::launch_local_h<double>(o.grid_plan(), dx.p + 2 * c0 * n, n, c, o.kinetic_device(),
+                                                 dh.p + 2 * c0 * n, n, t1, t2, s),
+                "local H");
+        else
+          check(sdft_hip::launch_local_h<float>(o.grid_plan(), dx.p + 2 * c0 * n, n, c, o.kinetic_device(),
+                                                dh.p + 2 * c0 * n, n, t1, t2, s),
+                "local H");
+      }
     check(hipEventRecord(e1, s), "event record");
     check(hipEventSynchronize(e1), "event sync");
     check(hipEventElapsedTime(&ms, e0, e1), "elapsed");

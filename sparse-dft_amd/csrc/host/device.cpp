@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <complex>
 #include <cstring>
 #include <limits>
@@ -408,10 +409,21 @@ DeviceOperator::DeviceOperator(Context& ctx, const OperatorDesc& d) : ctx_(ctx) 
     block_offset_.upload(boffs.data(), boffs.size(), s);
     ctx_.sync();
   }
-  // chunk of columns per launch: bounds the stick buffers (~512 MiB each)
-  const std::size_t per_col = std::size_t(grid_.dev.nsticks) * shape_[2] * 16;
-  chunk_ = std::max<std::size_t>(1, std::min<std::size_t>(512, (std::size_t(512) << 20) / std::max<std::size_t>(1, per_col)));
+  set_chunk();
   ctx_.sync();
+}
+
+// Columns per local-H launch: the stick buffers stay below 512 MiB each;
+// SDFT_HIP_APPLY_CHUNK lowers the cap (a smaller chunk keeps the stick
+// buffers resident in the last-level cache between the three kernels).
+void DeviceOperator::set_chunk() {
+  const std::size_t per_col = std::size_t(grid_.dev.nsticks) * shape_[2] * 16;
+  std::size_t cap = 512;
+  if (const char* e = std::getenv("SDFT_HIP_APPLY_CHUNK")) {
+    const long v = std::atol(e);
+    if (v > 0) cap = std::min<std::size_t>(cap, std::size_t(v));
+  }
+  chunk_ = std::max<std::size_t>(1, std::min<std::size_t>(cap, (std::size_t(512) << 20) / std::max<std::size_t>(1, per_col)));
 }
 
 DeviceOperator::DeviceOperator(Context& ctx, const SnapshotArrays& sa) : ctx_(ctx) {
@@ -479,8 +491,7 @@ DeviceOperator::DeviceOperator(Context& ctx, const SnapshotArrays& sa) : ctx_(ct
       onsite_q_.upload(reinterpret_cast<const double*>(sa.onsite_q.data()), 2 * sa.onsite_q.size(), s);
     }
   }
-  const std::size_t per_col = std::size_t(grid_.dev.nsticks) * shape_[2] * 16;
-  chunk_ = std::max<std::size_t>(1, std::min<std::size_t>(512, (std::size_t(512) << 20) / std::max<std::size_t>(1, per_col)));
+  set_chunk();
   ctx_.sync();
 }
 

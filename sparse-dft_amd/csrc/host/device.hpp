@@ -202,6 +202,7 @@ class DeviceOperator {
   // it. Returns the retained points; *retained gets the kept fraction of
   // sum_r sum_i |w_i(r)|^2. lattice: 3 rows of cell vectors (bohr);
   // fractional: atoms x 3; radius: atoms (any of them may be null).
+  void set_chunk();
   long enable_real_space_search(double factor, const double* lattice = nullptr, const double* fractional = nullptr,
                                 const double* radius = nullptr, double* retained = nullptr);
   bool real_space_search() const { return rs_enabled_; }
