@@ -63,6 +63,46 @@ def cpu_baseline(prob, iterations_fp64):
             "seconds_sample": t1 + t2}
 
 
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "si64_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch by kernel kind, from the committed PMC summary
+    (lines `traffic_bytes_per_launch <kind> <bytes>`); empty if absent."""
+    out = {}
+    if os.path.exists(TRAFFIC_FILE):
+        for line in open(TRAFFIC_FILE):
+            f = line.strip().strip("`").split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch":
+                out[f[1]] = float(f[2])
+    return out
+
+
+def copy_calibration(mib):
+    """Eight device-to-device copies of `mib` MiB through the HIP runtime's copy
+    kernel: the reference workload for scaling the FETCH_SIZE / WRITE_SIZE
+    counters to bytes (each copy reads and writes exactly mib MiB)."""
+    import ctypes as ct
+    hip = ct.CDLL("libamdhip64.so")
+    n = ct.c_size_t(mib << 20)
+    src, dst = ct.c_void_p(), ct.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ct.byref(p), n):
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(src, 1, n):
+        raise RuntimeError("hipMemset failed")
+    t0 = time.perf_counter()
+    for _ in range(8):
+        if hip.hipMemcpy(dst, src, n, 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    if hip.hipDeviceSynchronize():
+        raise RuntimeError("hipDeviceSynchronize failed")
+    dt = time.perf_counter() - t0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_mib": mib, "copies": 8, "seconds": dt}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,9 +118,14 @@ def main():
     ap.add_argument("--serving", type=int, default=0,
                     help="also run N serving requests (Si8..Si64 mix, Poisson arrivals) and report their p50/p99")
     ap.add_argument("--serving-workers", type=int, default=4)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 8 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     ap.add_argument("--real-space-factor", type=float, default=1.3,
                     help="also time a solve with the real-space projector search (0: skip)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by k-point/request "
               f"across GPUs with no exchange, see DESIGN.md", file=sys.stderr)
@@ -142,14 +187,19 @@ def main():
         ach = st["work"] / (st["ms"] * 1e-3) / scale
         kernels[k] = {"launches": st["launches"], "ms_total": st["ms"], "ms_per_launch": st["ms"] / st["launches"],
                       "work_per_launch": st["work"] / st["launches"], "achieved": ach, "unit": unit,
-                      "frac": ach / peak, "share_of_solve_wall": st["ms"] * 1e-3 / total}
+                      "frac": ach / peak, "share_of_solve_wall": st["ms"] * 1e-3 / total,
+                      "algorithmic_bytes_per_launch": st["bytes"] / st["launches"]}
     dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
     peak, unit, bound, _ = peaks[dom]
+    traffic = measured_traffic().get(dom)
     roofline = {"bound": bound, "achieved": kernels[dom]["achieved"], "peak": peak, "unit": unit,
-                "frac": kernels[dom]["frac"], "traffic": None, "kernel": names[dom],
+                "frac": kernels[dom]["frac"], "traffic": traffic, "kernel": names[dom],
                 "work_per_launch": kernels[dom]["work_per_launch"],
                 "work_unit": "flops (8 m n k)" if bound == "mfma" else "algorithmic bytes",
                 "ms_per_launch": kernels[dom]["ms_per_launch"], "launches": kernels[dom]["launches"],
+                "algorithmic_bytes_per_launch": kernels[dom]["algorithmic_bytes_per_launch"],
+                "traffic_source": "profiles/si64_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+                if traffic is not None else None,
                 "measured": "HIP events on the launch stream over the timed steps"}
     # isolated 160-column local-H launches (FP32 and FP64) for reference
     bench = op.benchmark(ROOTS, which="h", precision=1, repetitions=20)

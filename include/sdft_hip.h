@@ -58,6 +58,9 @@ int sdft_hip_synchronize(sdft_hip_context* ctx);
 #define SDFT_HIP_CLOCK_LOCAL_FP32 3
 int sdft_hip_kernel_clock(sdft_hip_context* ctx, int enable);
 int sdft_hip_kernel_clock_stats(sdft_hip_context* ctx, int kind, long* launches, double* milliseconds, double* work);
+/* Algorithmic HBM bytes summed over the same launches (GEMM: operands and
+ * result once each; local H: the per-launch byte count above). */
+int sdft_hip_kernel_clock_bytes(sdft_hip_context* ctx, int kind, double* bytes);
 
 /* Accelerator::gemm / gemm_immutable_left (src/compute_backend.hpp:67-73):
  * C = op(A) op(B) (+ C if add), ops 0 = Normal, 1 = Adjoint, host pointers.

@@ -158,12 +158,16 @@ template <int S, typename T>
 struct Dft<16, S, T> : DftSplit<16, 4, 4, S, T> {};
 
 // ------------------------------------------------------------ LDS line FFTs
-// Lines live in LDS: element i of line l is at base[l * ls + i * es].
+// Lines live in LDS: element i of line l is at base[l * ls + i * es], plus
+// `jump` for lines l >= split (two runs of lines in one pass; split = nlines
+// and jump = 0 for a single run).
 // tw[j] = (cos 2 pi j/L, sin 2 pi j/L) for j in [0, L).
 // `fast_lines`: consecutive threads take consecutive lines (use when ls == 1).
 struct LineGeom {
   int nlines, ls, es;
   bool fast_lines;
+  int split = 1 << 30, jump = 0;
+  __device__ __forceinline__ int line_offset(int line) const { return line * ls + (line >= split ? jump : 0); }
 };
 
 template <int NV, typename T>
@@ -188,7 +192,7 @@ __device__ __forceinline__ void lds_strided_pass(cx<T>* base, const LineGeom& g,
   for (int item = tid; item < items; item += nthreads) {
     int line, n1;
     item_split<L2, T>(item, L1, g, line, n1);
-    cx<T>* p = base + line * g.ls + n1 * g.es;
+    cx<T>* p = base + g.line_offset(line) + n1 * g.es;
     cx<T> v[L2];
 #pragma unroll
     for (int n2 = 0; n2 < L2; ++n2) v[n2] = p[n2 * L1 * g.es];
@@ -215,7 +219,7 @@ __device__ __forceinline__ void lds_group_pass(cx<T>* base, const LineGeom& g, c
   for (int item = tid; item < items; item += nthreads) {
     int line, b;
     item_split<L1, T>(item, L2, g, line, b);
-    cx<T>* p = base + line * g.ls + b * L1 * g.es;
+    cx<T>* p = base + g.line_offset(line) + b * L1 * g.es;
     cx<T> v[L1];
 #pragma unroll
     for (int a = 0; a < L1; ++a) v[a] = p[a * g.es];

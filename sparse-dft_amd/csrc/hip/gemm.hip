@@ -50,6 +50,25 @@ __device__ __forceinline__ void load2(const S* p, double& re, double& im) {
 // ones, so that no MFMA waits on the accumulator of the one just issued.
 // PF: the next K slab is loaded into registers while the current one is
 // multiplied from LDS.
+// XCD-aware tile order. Workgroups are dispatched round-robin over the eight
+// XCDs (linear id L runs on XCD L % 8), each with its own L2. The bijection
+// below hands every XCD a contiguous range of tiles in (split, m, n) order
+// with n fastest, so the tiles that share an A panel (and, for a few M tiles,
+// a B panel) run on the same XCD and reuse it from that XCD's L2.
+constexpr int kXcds = 8;
+struct TileId {
+  int m, n, split;
+};
+__device__ __forceinline__ TileId xcd_tile(int mt, int nt, int splits) {
+  const int total = mt * nt * splits;
+  const int lin = blockIdx.x + mt * (blockIdx.y + nt * blockIdx.z);
+  const int xcd = lin % kXcds, slot = lin / kXcds;
+  const int per = (total + kXcds - 1) / kXcds;  // tiles of the fuller XCDs
+  const int full = total - (per - 1) * kXcds;   // number of XCDs holding `per` tiles
+  const int t = xcd < full ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
+  return {(t / nt) % mt, t % nt, t / (nt * mt)};
+}
+
 // At least SDFT_GEMM_MIN_WAVES waves per SIMD: left alone, the FP64 tiles keep
 // their accumulators in AGPRs and the alpha/beta epilogue copies them all to
 // VGPRs at once, which caps occupancy at two waves.
@@ -68,8 +87,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDFT_GEMM_M
   __shared__ T bs_re[BK][TBN + PAD], bs_im[BK][TBN + PAD];
   using V = typename Acc<T>::type;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = blockIdx.x * TBM, n0 = blockIdx.y * TBN;
-  const int k0 = blockIdx.z * kps, k1 = min(K, k0 + kps);
+  const TileId tile = xcd_tile(gridDim.x, gridDim.y, gridDim.z);
+  const int m0 = tile.m * TBM, n0 = tile.n * TBN;
+  const int k0 = tile.split * kps, k1 = min(K, k0 + kps);
   const int wm = (wave % WGM) * WM, wn = (wave / WGM) * WN;
   V acc_re[IM][JN], acc_im[IM][JN];
 #pragma unroll
@@ -182,7 +202,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDFT_GEMM_M
         if (m >= M || n >= N) continue;
         double re = alpha * double(acc_re[i][j][r]), im = alpha * double(acc_im[i][j][r]);
         if (split) {
-          double* w = work + 2 * ((long)blockIdx.z * M * N + m + (long)n * M);
+          double* w = work + 2 * ((long)tile.split * M * N + m + (long)n * M);
           w[0] = re;
           w[1] = im;
         } else {

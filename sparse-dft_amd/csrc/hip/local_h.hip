@@ -96,8 +96,18 @@ __global__ __launch_bounds__(NT) void k_planes(const cx<T>* __restrict__ t1, cx<
   const int zs = blockIdx.x;
   const long col = blockIdx.y;
   const int np = n0 * n1;
-  // y lines: one per x, contiguous elements (two ranges of active x)
-  const LineGeom gy0{xl0, n1, 1, false}, gy1{xl1, n1, 1, false};
+  // twiddles of both line lengths staged in LDS after the plane
+  cx<T>* twx = plane + np;
+  cx<T>* twy = twx + n0;
+  for (int e = tid; e < n0; e += nt) twx[e] = tw_x[e];
+  for (int e = tid; e < n1; e += nt) twy[e] = tw_y[e];
+  // y lines: one per x, contiguous elements; the two ranges of active x run
+  // as one set of lines (x = xs0 .. xs0 + xl0 - 1, then xs1 .. xs1 + xl1 - 1)
+  LineGeom gy{xl0 + xl1, n1, 1, false};
+  gy.split = xl0;
+  gy.jump = (xs1 - xs0 - xl0) * n1;
+  cx<T>* ybase = plane + (xl0 ? xs0 : xs1) * n1;
+  if (!xl0) gy.split = 1 << 30;
   // x lines: one per y, elements strided by n1
   const LineGeom gx{n1, 1, n1, true};
   if (mode == 2) {
@@ -111,9 +121,8 @@ __global__ __launch_bounds__(NT) void k_planes(const cx<T>* __restrict__ t1, cx<
     const cx<T>* in = t1 + (col * n2 + zs) * (long)nsticks;
     for (int s = tid; s < nsticks; s += nt) plane[stick_xy[s]] = in[s];
     __syncthreads();
-    if (xl0) line_fft_dispatch<T, +1, true>(n1, plane + xs0 * n1, gy0, tw_y, tid, nt);
-    if (xl1) line_fft_dispatch<T, +1, true>(n1, plane + xs1 * n1, gy1, tw_y, tid, nt);
-    line_fft_dispatch<T, +1, true>(n0, plane, gx, tw_x, tid, nt);
+    if (xl0 + xl1) line_fft_dispatch<T, +1, true>(n1, ybase, gy, twy, tid, nt);
+    line_fft_dispatch<T, +1, true>(n0, plane, gx, twx, tid, nt);
     if (mode == 1) {
       cx<T>* out = t2 + (col * n2 + zs) * (long)np;
       for (int e = tid; e < np; e += nt) out[e] = plane[e];
@@ -127,11 +136,16 @@ __global__ __launch_bounds__(NT) void k_planes(const cx<T>* __restrict__ t1, cx<
     }
     __syncthreads();
   }
-  line_fft_dispatch<T, -1, false>(n0, plane, gx, tw_x, tid, nt);
-  if (xl0) line_fft_dispatch<T, -1, false>(n1, plane + xs0 * n1, gy0, tw_y, tid, nt);
-  if (xl1) line_fft_dispatch<T, -1, false>(n1, plane + xs1 * n1, gy1, tw_y, tid, nt);
+  line_fft_dispatch<T, -1, false>(n0, plane, gx, twx, tid, nt);
+  if (xl0 + xl1) line_fft_dispatch<T, -1, false>(n1, ybase, gy, twy, tid, nt);
   cx<T>* out = t2 + (col * n2 + zs) * (long)nsticks;
   for (int s = tid; s < nsticks; s += nt) out[s] = plane[stick_xy[s]];
+}
+
+// LDS of one plane workgroup: the plane and the x and y twiddles.
+template <typename T>
+size_t plane_lds_bytes(int n0, int n1) {
+  return (size_t(n0) * n1 + n0 + n1) * sizeof(cx<T>);
 }
 
 // Threads per plane workgroup (one xy-plane per workgroup): 512 unless
@@ -283,7 +297,7 @@ hipError_t launch_local_h(const GridPlanDev& p, const double* x, long ldx, long 
                           double* y, long ldy, void* t1, void* t2, hipStream_t stream) {
   if (columns <= 0) return hipSuccess;
   // one xy-plane per workgroup must fit the LDS of one workgroup
-  if (size_t(p.n0) * p.n1 * sizeof(cx<T>) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
+  if (plane_lds_bytes<T>(p.n0, p.n1) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
   const int spb = sticks_per_block(p.n2);
   const dim3 ga((p.nsticks + spb - 1) / spb, (unsigned)columns);
   const size_t lds_lines = size_t(spb) * p.n2 * sizeof(cx<T>);
@@ -293,7 +307,7 @@ hipError_t launch_local_h(const GridPlanDev& p, const double* x, long ldx, long 
   const T* v = reinterpret_cast<const T*>(sizeof(T) == 8 ? (const void*)p.v64 : (const void*)p.v32);
   hipLaunchKernelGGL(k_sticks_backward<T>, ga, dim3(256), lds_lines, stream, x, ldx, p.stick_g, p.nsticks, p.n2, spb,
                      twz, reinterpret_cast<cx<T>*>(t1));
-  const size_t lds_plane = size_t(p.n0) * p.n1 * sizeof(cx<T>);
+  const size_t lds_plane = plane_lds_bytes<T>(p.n0, p.n1);
   launch_k_planes<T>(dim3(p.n2, (unsigned)columns), lds_plane, stream,
                      reinterpret_cast<const cx<T>*>(t1), reinterpret_cast<cx<T>*>(t2), p.stick_xy, p.nsticks, p.n0,
                      p.n1, p.n2, twx, twy, v, 0, p.xstart[0], p.xlines[0], p.xstart[1], p.xlines[1]);
@@ -318,13 +332,13 @@ hipError_t launch_scale_copy(const double* x, long ldx, double* y, long ldy, lon
 // slot-ordered real potential. `full` is a plan whose sticks are every (x,y).
 hipError_t launch_potential_setup(const GridPlanDev& full, const double* grid, void* t1, void* t2, double* v64,
                                   float* v32, double* block_ratio, int ratio_blocks, hipStream_t stream) {
-  if (size_t(full.n0) * full.n1 * sizeof(cx<double>) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
+  if (plane_lds_bytes<double>(full.n0, full.n1) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
   const int spb = sticks_per_block(full.n2);
   const dim3 ga((full.nsticks + spb - 1) / spb, 1);
   const size_t lds_lines = size_t(spb) * full.n2 * sizeof(cx<double>);
   hipLaunchKernelGGL(k_sticks_backward<double>, ga, dim3(256), lds_lines, stream, grid, 0L, full.stick_g, full.nsticks,
                      full.n2, spb, reinterpret_cast<const cx<double>*>(full.tw64[2]), reinterpret_cast<cx<double>*>(t1));
-  const size_t lds_plane = size_t(full.n0) * full.n1 * sizeof(cx<double>);
+  const size_t lds_plane = plane_lds_bytes<double>(full.n0, full.n1);
   launch_k_planes<double>(dim3(full.n2, 1), lds_plane, stream,
                      reinterpret_cast<const cx<double>*>(t1), reinterpret_cast<cx<double>*>(t2), full.stick_xy,
                      full.nsticks, full.n0, full.n1, full.n2, reinterpret_cast<const cx<double>*>(full.tw64[0]),
@@ -388,13 +402,13 @@ __global__ void k_gather_support(const cx<double>* __restrict__ planes, int n0, 
 hipError_t launch_backward_planes(const GridPlanDev& p, const double* x, long ldx, long columns, void* t1, void* t2,
                                   hipStream_t stream) {
   if (columns <= 0) return hipSuccess;
-  if (size_t(p.n0) * p.n1 * sizeof(cx<double>) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
+  if (plane_lds_bytes<double>(p.n0, p.n1) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
   const int spb = sticks_per_block(p.n2);
   const dim3 ga((p.nsticks + spb - 1) / spb, (unsigned)columns);
   const size_t lds_lines = size_t(spb) * p.n2 * sizeof(cx<double>);
   hipLaunchKernelGGL(k_sticks_backward<double>, ga, dim3(256), lds_lines, stream, x, ldx, p.stick_g, p.nsticks, p.n2,
                      spb, reinterpret_cast<const cx<double>*>(p.tw64[2]), reinterpret_cast<cx<double>*>(t1));
-  const size_t lds_plane = size_t(p.n0) * p.n1 * sizeof(cx<double>);
+  const size_t lds_plane = plane_lds_bytes<double>(p.n0, p.n1);
   launch_k_planes<double>(dim3(p.n2, (unsigned)columns), lds_plane, stream,
                      reinterpret_cast<const cx<double>*>(t1), reinterpret_cast<cx<double>*>(t2), p.stick_xy,
                      p.nsticks, p.n0, p.n1, p.n2, reinterpret_cast<const cx<double>*>(p.tw64[0]),
@@ -407,7 +421,7 @@ template <typename T>
 hipError_t launch_backward_full(const GridPlanDev& p, const double* x, long ldx, long columns, void* t1, void* t2,
                                 hipStream_t stream) {
   if (columns <= 0) return hipSuccess;
-  if (size_t(p.n0) * p.n1 * sizeof(cx<T>) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
+  if (plane_lds_bytes<T>(p.n0, p.n1) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
   const int spb = sticks_per_block(p.n2);
   const dim3 ga((p.nsticks + spb - 1) / spb, (unsigned)columns);
   const size_t lds_lines = size_t(spb) * p.n2 * sizeof(cx<T>);
@@ -416,7 +430,7 @@ hipError_t launch_backward_full(const GridPlanDev& p, const double* x, long ldx,
   const cx<T>* twz = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[2] : p.tw32[2]);
   hipLaunchKernelGGL(k_sticks_backward<T>, ga, dim3(256), lds_lines, stream, x, ldx, p.stick_g, p.nsticks, p.n2, spb,
                      twz, reinterpret_cast<cx<T>*>(t1));
-  const size_t lds_plane = size_t(p.n0) * p.n1 * sizeof(cx<T>);
+  const size_t lds_plane = plane_lds_bytes<T>(p.n0, p.n1);
   launch_k_planes<T>(dim3(p.n2, (unsigned)columns), lds_plane, stream,
                      reinterpret_cast<const cx<T>*>(t1), reinterpret_cast<cx<T>*>(t2), p.stick_xy, p.nsticks, p.n0,
                      p.n1, p.n2, twx, twy, (const T*)nullptr, 1, p.xstart[0], p.xlines[0], p.xstart[1], p.xlines[1]);
@@ -427,11 +441,11 @@ template <typename T>
 hipError_t launch_forward_planes(const GridPlanDev& p, const void* planes, long columns, const double* x, long ldx,
                                  const double* diag, double* y, long ldy, void* t_sticks, hipStream_t stream) {
   if (columns <= 0) return hipSuccess;
-  if (size_t(p.n0) * p.n1 * sizeof(cx<T>) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
+  if (plane_lds_bytes<T>(p.n0, p.n1) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
   const cx<T>* twx = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[0] : p.tw32[0]);
   const cx<T>* twy = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[1] : p.tw32[1]);
   const cx<T>* twz = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[2] : p.tw32[2]);
-  const size_t lds_plane = size_t(p.n0) * p.n1 * sizeof(cx<T>);
+  const size_t lds_plane = plane_lds_bytes<T>(p.n0, p.n1);
   launch_k_planes<T>(dim3(p.n2, (unsigned)columns), lds_plane, stream,
                      reinterpret_cast<const cx<T>*>(planes), reinterpret_cast<cx<T>*>(t_sticks), p.stick_xy,
                      p.nsticks, p.n0, p.n1, p.n2, twx, twy, (const T*)nullptr, 2, p.xstart[0], p.xlines[0],

@@ -292,19 +292,39 @@ size_t lds_bytes(int n, int rpw) {
   return (size_t(rpw) * n + 3 * size_t(n) + 2 * size_t(rpw)) * sizeof(Z) + 2 * kWaves * sizeof(double) +
          size_t(rpw) * sizeof(int);
 }
+
+
+// LDS the kernel declares statically (the workgroup reductions of the
+// runtime included), which the dynamic slab must leave room for.
+size_t static_lds() {
+  static const size_t bytes = [] {
+    hipFuncAttributes attr{};
+    if (hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(k_hetrd_lds)) != hipSuccess) return size_t(4096);
+    return size_t(attr.sharedSizeBytes);
+  }();
+  return bytes;
+}
 }  // namespace
 
 bool hetrd_plan(int n, int* rows_per_group, int* groups) {
   if (n < 1 || n > kHetrdMaxOrder) return false;
   const int cus = device_cus();
-  const size_t budget = size_t(max_lds());
-  // smallest rows-per-group whose grid fits one workgroup per CU, then as many
-  // rows as the LDS holds (fewer workgroups make a cheaper barrier)
+  const size_t total = size_t(max_lds()), fixed = static_lds();
+  if (fixed >= total) return false;
+  const size_t budget = total - fixed;
+  // as many rows per workgroup as the LDS holds (fewer workgroups make a
+  // cheaper barrier), one workgroup per CU
   int rpw = 1;
   while (lds_bytes(n, rpw + 1) <= budget && rpw < n) ++rpw;
   if (lds_bytes(n, rpw) > budget) return false;
   const int g = (n + rpw - 1) / rpw;
   if (cus > 0 && g > cus) return false;
+  // every workgroup must be resident at once (cooperative launch)
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_hetrd_lds), kThreads,
+                                                   lds_bytes(n, rpw)) != hipSuccess ||
+      per_cu < 1 || (cus > 0 && long(per_cu) * cus < g))
+    return false;
   *rows_per_group = rpw;
   *groups = g;
   return true;

@@ -120,6 +120,13 @@ int sdft_hip_kernel_clock(sdft_hip_context* ctx, int enable) {
   });
 }
 
+int sdft_hip_kernel_clock_bytes(sdft_hip_context* ctx, int kind, double* bytes) {
+  return guarded([&] {
+    if (!ctx || !bytes) throw std::invalid_argument("HIP: null kernel clock argument");
+    *bytes = ctx->impl.clock_stat(kind).bytes;
+  });
+}
+
 int sdft_hip_kernel_clock_stats(sdft_hip_context* ctx, int kind, long* launches, double* milliseconds, double* work) {
   return guarded([&] {
     if (!ctx || !launches || !milliseconds || !work) throw std::invalid_argument("HIP: null kernel clock argument");

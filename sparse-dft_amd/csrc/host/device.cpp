@@ -192,7 +192,7 @@ void Context::clock_enable(bool on) {
   for (auto& c : clock_) c = ClockStat{};
 }
 
-LaunchEvents Context::clock_launch(int kind, double work) {
+LaunchEvents Context::clock_launch(int kind, double work, double bytes) {
   if (!clock_on_) return {};
   if (kind < 0 || kind >= kClockKinds) throw std::invalid_argument("HIP: invalid kernel clock kind");
   while (events_.size() < events_used_ + 2) {
@@ -200,7 +200,7 @@ LaunchEvents Context::clock_launch(int kind, double work) {
     check(hipEventCreate(&e), "hipEventCreate");
     events_.push_back(e);
   }
-  pending_.push_back({kind, work, events_used_});
+  pending_.push_back({kind, work, bytes, events_used_});
   LaunchEvents ev{events_[events_used_], events_[events_used_ + 1]};
   events_used_ += 2;
   return ev;
@@ -216,10 +216,17 @@ Context::ClockStat Context::clock_stat(int kind) {
     c.launches += 1;
     c.milliseconds += ms;
     c.work += p.work;
+    c.bytes += p.bytes;
   }
   pending_.clear();
   events_used_ = 0;
   return clock_[kind];
+}
+
+// Algorithmic HBM bytes of one complex GEMM: A (M x K, a_bytes per element as
+// stored), B (K x N, FP64 complex) and C (M x N) once each, plus D when beta != 0.
+double gemm_algorithmic_bytes(long M, long N, long K, int a_bytes, bool reads_d) {
+  return double(M) * K * a_bytes + 16.0 * double(K) * N + 16.0 * double(M) * N * (reads_d ? 2 : 1);
 }
 
 void Context::zgemm(Precision prec, int aop, int bop, long M, long N, long K, double alpha, const double* A, long lda,
@@ -229,7 +236,8 @@ void Context::zgemm(Precision prec, int aop, int bop, long M, long N, long K, do
     throw std::overflow_error("HIP: GEMM dimension overflow");
   const std::size_t bytes = zgemm_workspace_bytes(int(M), int(N), int(K));
   double* w = bytes ? gemm_work(bytes) : nullptr;
-  const LaunchEvents ev = K > 0 ? clock_launch(prec == Precision::FP64 ? kGemmFP64 : kGemmFP32, 8.0 * M * N * K)
+  const LaunchEvents ev = K > 0 ? clock_launch(prec == Precision::FP64 ? kGemmFP64 : kGemmFP32, 8.0 * M * N * K,
+                                               gemm_algorithmic_bytes(M, N, K, 16, beta != 0.0))
                                 : LaunchEvents{};
   check(sdft_hip::zgemm(prec, aop, bop, int(M), int(N), int(K), alpha, A, lda, B, ldb, C, ldc, D, ldd, beta, w,
                         bytes ? gemm_work_bytes() : 0, stream_, &ev),
@@ -828,7 +836,7 @@ std::vector<std::complex<double>> DeviceOperator::density(const double* x, long 
     if (len > 128) throw std::invalid_argument("HIP: density grid beyond the device FFT lengths");
     shape[a] = len;
   }
-  if (std::size_t(shape[0]) * shape[1] * 16 > std::size_t(max_lds_bytes()))
+  if ((std::size_t(shape[0]) * shape[1] + shape[0] + shape[1]) * 16 > std::size_t(max_lds_bytes()))
     throw std::invalid_argument("HIP: density grid plane exceeds the LDS of one workgroup");
   const long count = long(shape[0]) * shape[1] * shape[2];
   std::vector<std::uint64_t> idx(n_);
@@ -918,7 +926,8 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
     else {
       const std::size_t bytes = zgemm_workspace_bytes(int(np), int(c), int(n));
       double* wk = bytes ? ctx_.gemm_work(bytes) : nullptr;
-      const LaunchEvents ev = ctx_.clock_launch(Context::kGemmFP32, 8.0 * np * c * n);
+      const LaunchEvents ev =
+          ctx_.clock_launch(Context::kGemmFP32, 8.0 * np * c * n, gemm_algorithmic_bytes(np, c, n, 8, false));
       check(zgemm_f32_storage(0, 0, int(np), int(c), int(n), 1.0, b32_.p, np, x, n, p_.p, np, nullptr, 0, 0.0, wk,
                               bytes ? ctx_.gemm_work_bytes() : 0, s, &ev),
             "projection");
@@ -933,8 +942,8 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
     t1_.reserve(bytes);
     t2_.reserve(bytes);
     const bool f64 = prec == Precision::FP64;
-    const LaunchEvents ev = ctx_.clock_launch(f64 ? Context::kLocalFP64 : Context::kLocalFP32,
-                                              local_algorithmic_bytes(n, count_, c, f64));
+    const double local_bytes = local_algorithmic_bytes(n, count_, c, f64);
+    const LaunchEvents ev = ctx_.clock_launch(f64 ? Context::kLocalFP64 : Context::kLocalFP32, local_bytes, local_bytes);
     if (ev.begin) check(hipEventRecord(ev.begin, s), "hipEventRecord");
     if (f64)
       check(launch_local_h<double>(grid_.dev, x, n, c, kinetic_d_.p, hy, n, t1_.p, t2_.p, s), "local H");
@@ -948,7 +957,8 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
     else {
       const std::size_t bytes = zgemm_workspace_bytes(int(n), int(c), int(np));
       double* wk = bytes ? ctx_.gemm_work(bytes) : nullptr;
-      const LaunchEvents ev = ctx_.clock_launch(Context::kGemmFP32, 8.0 * n * c * np);
+      const LaunchEvents ev =
+          ctx_.clock_launch(Context::kGemmFP32, 8.0 * n * c * np, gemm_algorithmic_bytes(n, c, np, 8, beta != 0.0));
       check(zgemm_f32_storage(1, 0, int(n), int(c), int(np), 1.0, b32_.p, np, w, np, out, n, d, n, beta, wk,
                               bytes ? ctx_.gemm_work_bytes() : 0, s, &ev),
             "back projection");

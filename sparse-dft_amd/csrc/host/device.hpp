@@ -23,6 +23,8 @@ namespace sdft_hip {
 void check(hipError_t e, const char* what);
 void check(rocblas_status s, const char* what);
 
+double gemm_algorithmic_bytes(long M, long N, long K, int a_bytes, bool reads_d);
+
 // Grow-only device buffer (src/compute_cuda.cu:72-79 reserve semantics).
 template <class T>
 struct DevBuf {
@@ -101,10 +103,11 @@ class Context {
   struct ClockStat {
     long launches = 0;
     double milliseconds = 0, work = 0;  // work: flops (GEMM) or algorithmic bytes (local H)
+    double bytes = 0;                     // algorithmic HBM bytes (operands read once, result written once)
   };
   void clock_enable(bool on);  // clears the sums either way
   // Events to record for one launch (both null when the clock is off).
-  LaunchEvents clock_launch(int kind, double work);
+  LaunchEvents clock_launch(int kind, double work, double bytes);
   ClockStat clock_stat(int kind);  // synchronises the stream and folds pending pairs
 
   // caches for the drop-in host-pointer entry points (identity tokens)
@@ -141,7 +144,7 @@ class Context {
   std::size_t events_used_ = 0;
   struct PendingLaunch {
     int kind;
-    double work;
+    double work, bytes;
     std::size_t first_event;
   };
   std::vector<PendingLaunch> pending_;

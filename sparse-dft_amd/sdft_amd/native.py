@@ -167,14 +167,16 @@ class Context:
         _check(lib().sdft_hip_kernel_clock(self.handle, ct.c_int(int(bool(enable)))))
 
     def kernel_clock_stats(self):
-        """{kind: {launches, ms, work}}: event-timed sums since the clock was started
-        (work in flops for GEMM kinds, algorithmic bytes for local-H kinds)."""
+        """{kind: {launches, ms, work, bytes}}: event-timed sums since the clock was
+        started (work in flops for GEMM kinds, algorithmic bytes for local-H kinds;
+        bytes: algorithmic HBM bytes)."""
         out = {}
         for name, kind in self.CLOCK_KINDS.items():
-            n, ms, work = ct.c_long(), ct.c_double(), ct.c_double()
+            n, ms, work, nbytes = ct.c_long(), ct.c_double(), ct.c_double(), ct.c_double()
             _check(lib().sdft_hip_kernel_clock_stats(self.handle, ct.c_int(kind), ct.byref(n), ct.byref(ms),
                                                      ct.byref(work)))
-            out[name] = {"launches": n.value, "ms": ms.value, "work": work.value}
+            _check(lib().sdft_hip_kernel_clock_bytes(self.handle, ct.c_int(kind), ct.byref(nbytes)))
+            out[name] = {"launches": n.value, "ms": ms.value, "work": work.value, "bytes": nbytes.value}
         return out
 
     # ---- Accelerator mirror (host pointers)
