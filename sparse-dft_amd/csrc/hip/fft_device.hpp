@@ -162,17 +162,22 @@ struct Dft<16, S, T> : DftSplit<16, 4, 4, S, T> {};
 // `jump` for lines l >= split (two runs of lines in one pass; split = nlines
 // and jump = 0 for a single run).
 // tw[j] = (cos 2 pi j/L, sin 2 pi j/L) for j in [0, L).
-// `fast_lines`: consecutive threads take consecutive lines (use when ls == 1).
+// `fast_lines`: consecutive threads take consecutive lines in both passes
+// (use when ls == 1). `fast_group`: the same in the digit-group pass only; with
+// contiguous lines (es == 1) of odd stride ls this keeps that pass free of LDS
+// bank conflicts (a thread of the group pass reads L1 consecutive elements, so
+// threads of one line would sit L1 elements apart).
 struct LineGeom {
   int nlines, ls, es;
   bool fast_lines;
   int split = 1 << 30, jump = 0;
+  bool fast_group = false;
   __device__ __forceinline__ int line_offset(int line) const { return line * ls + (line >= split ? jump : 0); }
 };
 
 template <int NV, typename T>
-__device__ __forceinline__ void item_split(int item, int per_line, const LineGeom& g, int& line, int& idx) {
-  if (g.fast_lines) {
+__device__ __forceinline__ void item_split(int item, int per_line, const LineGeom& g, bool fast, int& line, int& idx) {
+  if (fast) {
     line = item % g.nlines;
     idx = item / g.nlines;
   } else {
@@ -191,7 +196,7 @@ __device__ __forceinline__ void lds_strided_pass(cx<T>* base, const LineGeom& g,
   const int items = g.nlines * L1;
   for (int item = tid; item < items; item += nthreads) {
     int line, n1;
-    item_split<L2, T>(item, L1, g, line, n1);
+    item_split<L2, T>(item, L1, g, g.fast_lines, line, n1);
     cx<T>* p = base + g.line_offset(line) + n1 * g.es;
     cx<T> v[L2];
 #pragma unroll
@@ -218,7 +223,7 @@ __device__ __forceinline__ void lds_group_pass(cx<T>* base, const LineGeom& g, c
   const int items = g.nlines * L2;
   for (int item = tid; item < items; item += nthreads) {
     int line, b;
-    item_split<L1, T>(item, L2, g, line, b);
+    item_split<L1, T>(item, L2, g, g.fast_lines || g.fast_group, line, b);
     cx<T>* p = base + g.line_offset(line) + b * L1 * g.es;
     cx<T> v[L1];
 #pragma unroll

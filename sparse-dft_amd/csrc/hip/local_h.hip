@@ -24,6 +24,11 @@
 
 namespace sdft_hip {
 
+// LDS stride of contiguous lines (z-sticks, plane rows): odd, so that threads
+// one line apart hit different LDS banks.
+__host__ __device__ inline int line_stride(int n) { return n | 1; }
+
+
 namespace {
 
 template <typename T, int S, bool TO_SLOTS>
@@ -64,18 +69,20 @@ __global__ __launch_bounds__(256) void k_sticks_backward(const double* __restric
   const int nl = min(spb, nsticks - first);
   const long col = blockIdx.y;
   const double* xc = x + 2 * col * ldx;
+  const int ls = line_stride(n2);
   for (int e = tid; e < nl * n2; e += nt) {
     const int s = e / n2, z = e - s * n2;
     const int g = stick_g[(long)(first + s) * n2 + z];
-    lines[e] = g >= 0 ? load_cx<T>(xc + 2L * g) : cx<T>{T(0), T(0)};
+    lines[s * ls + z] = g >= 0 ? load_cx<T>(xc + 2L * g) : cx<T>{T(0), T(0)};
   }
   __syncthreads();
-  LineGeom geom{nl, n2, 1, false};
+  LineGeom geom{nl, ls, 1, false};
+  geom.fast_group = true;
   line_fft_dispatch<T, +1, true>(n2, lines, geom, tw_z, tid, nt);
   cx<T>* out = t1 + col * (long)n2 * nsticks;
   for (int e = tid; e < nl * n2; e += nt) {
     const int zs = e / nl, s = e - zs * nl;
-    out[(long)zs * nsticks + first + s] = lines[s * n2 + zs];
+    out[(long)zs * nsticks + first + s] = lines[s * ls + zs];
   }
 }
 
@@ -96,56 +103,66 @@ __global__ __launch_bounds__(NT) void k_planes(const cx<T>* __restrict__ t1, cx<
   const int zs = blockIdx.x;
   const long col = blockIdx.y;
   const int np = n0 * n1;
+  // plane rows (fixed x) of odd stride n1p in LDS: element (x, y) at x n1p + y
+  const int n1p = line_stride(n1), pad = n1p - n1;
   // twiddles of both line lengths staged in LDS after the plane
-  cx<T>* twx = plane + np;
+  cx<T>* twx = plane + n0 * n1p;
   cx<T>* twy = twx + n0;
   for (int e = tid; e < n0; e += nt) twx[e] = tw_x[e];
   for (int e = tid; e < n1; e += nt) twy[e] = tw_y[e];
   // y lines: one per x, contiguous elements; the two ranges of active x run
   // as one set of lines (x = xs0 .. xs0 + xl0 - 1, then xs1 .. xs1 + xl1 - 1)
-  LineGeom gy{xl0 + xl1, n1, 1, false};
+  LineGeom gy{xl0 + xl1, n1p, 1, false};
+  gy.fast_group = true;
   gy.split = xl0;
-  gy.jump = (xs1 - xs0 - xl0) * n1;
-  cx<T>* ybase = plane + (xl0 ? xs0 : xs1) * n1;
+  gy.jump = (xs1 - xs0 - xl0) * n1p;
+  cx<T>* ybase = plane + (xl0 ? xs0 : xs1) * n1p;
   if (!xl0) gy.split = 1 << 30;
-  // x lines: one per y, elements strided by n1
-  const LineGeom gx{n1, 1, n1, true};
+  // x lines: one per y, elements strided by n1p
+  const LineGeom gx{n1, 1, n1p, true};
   if (mode == 2) {
     // start from a full slot-order plane (real-space search images)
     const cx<T>* inp = t1 + (col * n2 + zs) * (long)np;
-    for (int e = tid; e < np; e += nt) plane[e] = inp[e];
+    for (int e = tid; e < np; e += nt) plane[e + (e / n1) * pad] = inp[e];
     __syncthreads();
   } else {
-    for (int e = tid; e < np; e += nt) plane[e] = {T(0), T(0)};
+    for (int e = tid; e < n0 * n1p; e += nt) plane[e] = {T(0), T(0)};
     __syncthreads();
     const cx<T>* in = t1 + (col * n2 + zs) * (long)nsticks;
-    for (int s = tid; s < nsticks; s += nt) plane[stick_xy[s]] = in[s];
+    for (int s = tid; s < nsticks; s += nt) {
+      const int sx = stick_xy[s];
+      plane[sx + (sx / n1) * pad] = in[s];
+    }
     __syncthreads();
     if (xl0 + xl1) line_fft_dispatch<T, +1, true>(n1, ybase, gy, twy, tid, nt);
     line_fft_dispatch<T, +1, true>(n0, plane, gx, twx, tid, nt);
     if (mode == 1) {
       cx<T>* out = t2 + (col * n2 + zs) * (long)np;
-      for (int e = tid; e < np; e += nt) out[e] = plane[e];
+      for (int e = tid; e < np; e += nt) out[e] = plane[e + (e / n1) * pad];
       return;
     }
     const T* v = v_slots + (long)zs * np;
     for (int e = tid; e < np; e += nt) {
       const T w = v[e];
-      plane[e].x *= w;
-      plane[e].y *= w;
+      cx<T>& q = plane[e + (e / n1) * pad];
+      q.x *= w;
+      q.y *= w;
     }
     __syncthreads();
   }
   line_fft_dispatch<T, -1, false>(n0, plane, gx, twx, tid, nt);
   if (xl0 + xl1) line_fft_dispatch<T, -1, false>(n1, ybase, gy, twy, tid, nt);
   cx<T>* out = t2 + (col * n2 + zs) * (long)nsticks;
-  for (int s = tid; s < nsticks; s += nt) out[s] = plane[stick_xy[s]];
+  for (int s = tid; s < nsticks; s += nt) {
+    const int sx = stick_xy[s];
+    out[s] = plane[sx + (sx / n1) * pad];
+  }
 }
 
 // LDS of one plane workgroup: the plane and the x and y twiddles.
 template <typename T>
 size_t plane_lds_bytes(int n0, int n1) {
-  return (size_t(n0) * n1 + n0 + n1) * sizeof(cx<T>);
+  return (size_t(n0) * line_stride(n1) + n0 + n1) * sizeof(cx<T>);
 }
 
 // Threads per plane workgroup (one xy-plane per workgroup): 512 unless
@@ -187,12 +204,14 @@ __global__ __launch_bounds__(256) void k_sticks_forward(const cx<T>* __restrict_
   const int nl = min(spb, nsticks - first);
   const long col = blockIdx.y;
   const cx<T>* in = t2 + col * (long)n2 * nsticks;
+  const int ls = line_stride(n2);
   for (int e = tid; e < nl * n2; e += nt) {
     const int zs = e / nl, s = e - zs * nl;
-    lines[s * n2 + zs] = in[(long)zs * nsticks + first + s];
+    lines[s * ls + zs] = in[(long)zs * nsticks + first + s];
   }
   __syncthreads();
-  LineGeom geom{nl, n2, 1, false};
+  LineGeom geom{nl, ls, 1, false};
+  geom.fast_group = true;
   line_fft_dispatch<T, -1, false>(n2, lines, geom, tw_z, tid, nt);
   const double* xc = x + 2 * col * ldx;
   double* yc = y + 2 * col * ldy;
@@ -202,7 +221,7 @@ __global__ __launch_bounds__(256) void k_sticks_forward(const cx<T>* __restrict_
     if (g < 0) continue;
     const double t = kinetic[g];
     const double2 xi = *reinterpret_cast<const double2*>(xc + 2L * g);
-    const cx<T> w = lines[e];
+    const cx<T> w = lines[s * ls + z];
     double2 r;
     r.x = t * xi.x + double(w.x) / count;
     r.y = t * xi.y + double(w.y) / count;
@@ -300,7 +319,7 @@ hipError_t launch_local_h(const GridPlanDev& p, const double* x, long ldx, long 
   if (plane_lds_bytes<T>(p.n0, p.n1) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
   const int spb = sticks_per_block(p.n2);
   const dim3 ga((p.nsticks + spb - 1) / spb, (unsigned)columns);
-  const size_t lds_lines = size_t(spb) * p.n2 * sizeof(cx<T>);
+  const size_t lds_lines = size_t(spb) * line_stride(p.n2) * sizeof(cx<T>);
   const cx<T>* twx = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[0] : p.tw32[0]);
   const cx<T>* twy = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[1] : p.tw32[1]);
   const cx<T>* twz = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[2] : p.tw32[2]);
@@ -335,7 +354,7 @@ hipError_t launch_potential_setup(const GridPlanDev& full, const double* grid, v
   if (plane_lds_bytes<double>(full.n0, full.n1) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
   const int spb = sticks_per_block(full.n2);
   const dim3 ga((full.nsticks + spb - 1) / spb, 1);
-  const size_t lds_lines = size_t(spb) * full.n2 * sizeof(cx<double>);
+  const size_t lds_lines = size_t(spb) * line_stride(full.n2) * sizeof(cx<double>);
   hipLaunchKernelGGL(k_sticks_backward<double>, ga, dim3(256), lds_lines, stream, grid, 0L, full.stick_g, full.nsticks,
                      full.n2, spb, reinterpret_cast<const cx<double>*>(full.tw64[2]), reinterpret_cast<cx<double>*>(t1));
   const size_t lds_plane = plane_lds_bytes<double>(full.n0, full.n1);
@@ -405,7 +424,7 @@ hipError_t launch_backward_planes(const GridPlanDev& p, const double* x, long ld
   if (plane_lds_bytes<double>(p.n0, p.n1) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
   const int spb = sticks_per_block(p.n2);
   const dim3 ga((p.nsticks + spb - 1) / spb, (unsigned)columns);
-  const size_t lds_lines = size_t(spb) * p.n2 * sizeof(cx<double>);
+  const size_t lds_lines = size_t(spb) * line_stride(p.n2) * sizeof(cx<double>);
   hipLaunchKernelGGL(k_sticks_backward<double>, ga, dim3(256), lds_lines, stream, x, ldx, p.stick_g, p.nsticks, p.n2,
                      spb, reinterpret_cast<const cx<double>*>(p.tw64[2]), reinterpret_cast<cx<double>*>(t1));
   const size_t lds_plane = plane_lds_bytes<double>(p.n0, p.n1);
@@ -424,7 +443,7 @@ hipError_t launch_backward_full(const GridPlanDev& p, const double* x, long ldx,
   if (plane_lds_bytes<T>(p.n0, p.n1) > size_t(max_lds_bytes())) return hipErrorInvalidConfiguration;
   const int spb = sticks_per_block(p.n2);
   const dim3 ga((p.nsticks + spb - 1) / spb, (unsigned)columns);
-  const size_t lds_lines = size_t(spb) * p.n2 * sizeof(cx<T>);
+  const size_t lds_lines = size_t(spb) * line_stride(p.n2) * sizeof(cx<T>);
   const cx<T>* twx = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[0] : p.tw32[0]);
   const cx<T>* twy = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[1] : p.tw32[1]);
   const cx<T>* twz = reinterpret_cast<const cx<T>*>(sizeof(T) == 8 ? p.tw64[2] : p.tw32[2]);
@@ -452,7 +471,7 @@ hipError_t launch_forward_planes(const GridPlanDev& p, const void* planes, long 
                      p.xstart[1], p.xlines[1]);
   const int spb = sticks_per_block(p.n2);
   const dim3 ga((p.nsticks + spb - 1) / spb, (unsigned)columns);
-  const size_t lds_lines = size_t(spb) * p.n2 * sizeof(cx<T>);
+  const size_t lds_lines = size_t(spb) * line_stride(p.n2) * sizeof(cx<T>);
   hipLaunchKernelGGL(k_sticks_forward<T>, ga, dim3(256), lds_lines, stream, reinterpret_cast<const cx<T>*>(t_sticks),
                      p.stick_g, p.nsticks, p.n2, spb, twz, x, ldx, diag, y, ldy, 0.0, double(p.count));
   return hipGetLastError();

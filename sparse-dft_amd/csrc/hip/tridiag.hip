@@ -5,14 +5,17 @@
 // zunmtr complete the decomposition.
 //
 // Layout: every workgroup keeps the rows g, g + G, g + 2G, ... of the matrix in
-// LDS for the whole reduction (the matrix never returns to HBM). Step k:
-//   1. every workgroup forms the reflector of the current column redundantly
-//      (identical code and data, so identical bits everywhere);
-//   2. p_i = sum_j A_ij v_j on its own rows, published together with the
-//      pre-update A_{i,k+1};
-//   3. one grid barrier;
-//   4. w = tau p - (tau/2)(tau p)^H v v, the rank-2 update of its rows, and the
-//      next column for all rows from the published A_{i,k+1}.
+// LDS for the whole reduction (the matrix never returns to HBM). Step k starts
+// with reflector k formed and p_i = sum_j A_ij v_j staged for the own rows,
+// together with the pre-update A_{i,k+1}:
+//   1. publication of the staged values and one grid barrier;
+//   2. every workgroup loads all published values at once and reduces
+//      (tau p)^H v;
+//   3. w = tau p - (tau/2)(tau p)^H v v (in registers), the next column for all
+//      rows from the published A_{i,k+1}, the rank-2 update of the own rows;
+//   4. reflector k + 1 from the new column, formed redundantly by every
+//      workgroup (identical code and data, so identical bits everywhere), and
+//      the products of step k + 1 on the updated rows.
 // The published vectors are double-buffered by step parity, so one barrier per
 // column suffices. Each workgroup raises its own arrival flag (one cache line
 // per workgroup, no read-modify-write contention) and polls all flags with one
@@ -134,6 +137,68 @@ __device__ bool publish_and_wait(unsigned* flags, int* abort_flag, unsigned step
   return __syncthreads_and(ok) != 0;
 }
 
+// Reflector of the column held in `col` at rows c0 + 1 .. n - 1 (LAPACK zlarfg
+// conventions, m = n - c0 - 1 entries): writes v[0..m) (v[0] = 1) and, for
+// workgroup 0, d[c0], e[c0], tau[c0] and the reflector tail. `xn2` is the sum
+// of |col[j]|^2 over j >= c0 + 2. Returns tau. Ends with a block barrier.
+__device__ Z form_reflector(const Z* col, int c0, int n, double xn2, Z* v, Z* __restrict__ refl, long ldr,
+                            double* __restrict__ d, double* __restrict__ e, Z* __restrict__ tau) {
+  const int tid = threadIdx.x, m = n - c0 - 1;
+  const Z alpha = col[c0 + 1];
+  Z t{0., 0.}, scale{0., 0.};
+  double beta;
+  if (xn2 == 0. && alpha.y == 0.) {
+    beta = alpha.x;
+  } else {
+    const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+    beta = alpha.x >= 0. ? -nrm : nrm;
+    t = {(beta - alpha.x) / beta, -alpha.y / beta};
+    const Z den{alpha.x - beta, alpha.y};  // 1 / (alpha - beta)
+    const double dd = den.x * den.x + den.y * den.y;
+    scale = {den.x / dd, -den.y / dd};
+  }
+  for (int j = tid; j < m; j += kThreads) v[j] = j == 0 ? Z{1., 0.} : zmul(col[c0 + 1 + j], scale);
+  if (blockIdx.x == 0) {
+    if (tid == 0) {
+      d[c0] = col[c0].x;
+      e[c0] = beta;
+      tau[c0] = t;
+    }
+    for (int j = 2 + tid; j <= m; j += kThreads) refl[(c0 + j) + size_t(c0) * ldr] = zmul(col[c0 + j], scale);
+  }
+  __syncthreads();
+  return t;
+}
+
+// p_i = sum_j A_ij v_j over the trailing columns c0 + 1 .. n - 1 for the own
+// rows i > c0 (one wave per row), staged in LDS with A_{i,c0+1}; returns the
+// number of staged rows through `staged`. Ends with a block barrier.
+__device__ void stage_products(const Z* rows, int n, int rpw, int c0, const Z* v, Z* stage_p, Z* stage_c,
+                               int* stage_i, int* staged) {
+  const int tid = threadIdx.x, lane = tid & 63, g = blockIdx.x, G = gridDim.x, m = n - c0 - 1;
+  const int first = c0 + 1 - g <= 0 ? 0 : (c0 + 1 - g + G - 1) / G;
+  for (int r = first + (tid >> 6); r < rpw; r += kWaves) {
+    const int i = g + r * G;
+    if (i >= n) break;
+    const Z* row = rows + size_t(r) * n + (c0 + 1);
+    Z acc{0., 0.};
+    for (int j = lane; j < m; j += 64) acc = zadd(acc, zmul(row[j], v[j]));
+    acc.x = wave_sum(acc.x);
+    acc.y = wave_sum(acc.y);
+    if (lane == 0) {
+      stage_p[r - first] = acc;
+      stage_c[r - first] = row[0];
+      stage_i[r - first] = i;
+    }
+  }
+  if (tid == 0) {
+    int cnt = 0;
+    for (int r = first; r < rpw && g + r * G < n; ++r) ++cnt;
+    *staged = cnt;
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(kThreads) void k_hetrd_lds(const Z* __restrict__ a, long lda, int n, int rpw,
                                                         Z* __restrict__ refl, long ldr, double* __restrict__ d,
                                                         double* __restrict__ e, Z* __restrict__ tau,
@@ -141,15 +206,13 @@ __global__ __launch_bounds__(kThreads) void k_hetrd_lds(const Z* __restrict__ a,
                                                         unsigned* __restrict__ flags, int* __restrict__ abort_flag) {
   extern __shared__ double lds_raw[];
   Z* rows = reinterpret_cast<Z*>(lds_raw);  // rpw x n
-  Z* v = rows + size_t(rpw) * n;            // reflector (trailing part), n
-  Z* w = v + n;                             // p, then w
-  Z* col = w + n;                           // current column, n
+  Z* vbuf = rows + size_t(rpw) * n;         // two reflectors (steps of both parities), n each
+  Z* col = vbuf + 2 * size_t(n);            // current column, n
   double* scratch = reinterpret_cast<double*>(col + n);  // 2 * kWaves
   Z* stage_p = reinterpret_cast<Z*>(scratch + 2 * kWaves);  // rpw staged p_i
   Z* stage_c = stage_p + rpw;                               // rpw staged A_{i,k+1}
   int* stage_i = reinterpret_cast<int*>(stage_c + rpw);     // rpw row indices
   __shared__ int staged;
-  __shared__ Z sh_a2;
 
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
   // row i of the Hermitian input is the conjugate of column i (coalesced read)
@@ -160,112 +223,81 @@ __global__ __launch_bounds__(kThreads) void k_hetrd_lds(const Z* __restrict__ a,
   }
   for (int j = tid; j < n; j += kThreads) col[j] = a[j];
   __syncthreads();
-
+  // prologue: reflector 0 and the products of step 0
+  Z t;
+  {
+    double part = 0.;
+    for (int j = 2 + tid; j < n; j += kThreads) part += col[j].x * col[j].x + col[j].y * col[j].y;
+    const double xn2 = block_sum(part, scratch);
+    t = form_reflector(col, 0, n, xn2, vbuf, refl, ldr, d, e, tau);
+    stage_products(rows, n, rpw, 0, vbuf, stage_p, stage_c, stage_i, &staged);
+  }
   for (int k = 0; k + 1 < n; ++k) {
     const int m = n - k - 1;  // order of the trailing block
-    // ---- 1. reflector of column k (LAPACK zlarfg conventions)
-    double part = 0.;
-    for (int j = k + 2 + tid; j < n; j += kThreads) part += col[j].x * col[j].x + col[j].y * col[j].y;
-    const double xn2 = block_sum(part, scratch);
-    const Z alpha = col[k + 1];
-    Z t{0., 0.}, scale{0., 0.};
-    double beta;
-    if (xn2 == 0. && alpha.y == 0.) {
-      beta = alpha.x;
-    } else {
-      const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-      beta = alpha.x >= 0. ? -nrm : nrm;
-      t = {(beta - alpha.x) / beta, -alpha.y / beta};
-      // 1 / (alpha - beta)
-      const Z den{alpha.x - beta, alpha.y};
-      const double dd = den.x * den.x + den.y * den.y;
-      scale = {den.x / dd, -den.y / dd};
-    }
-    for (int j = tid; j < m; j += kThreads) v[j] = j == 0 ? Z{1., 0.} : zmul(col[k + 1 + j], scale);
-    if (g == 0 && tid == 0) {
-      d[k] = col[k].x;
-      e[k] = beta;
-      tau[k] = t;
-    }
-    __syncthreads();
-    if (g == 0)
-      for (int j = 2 + tid; j <= m; j += kThreads) refl[(k + j) + size_t(k) * ldr] = v[j - 1];
-
-    // ---- 2. p_i = sum_j A_ij v_j on own trailing rows, staged in LDS with the
-    // pre-update A_{i,k+1}; the active rows are r >= first (rows g + r G > k)
+    Z* v = vbuf + size_t(k & 1) * n;
+    Z* vn = vbuf + size_t((k + 1) & 1) * n;
     Z* pp = pub_p + size_t(k & 1) * n;
     Z* pc = pub_c + size_t(k & 1) * n;
     const int first = k + 1 - g <= 0 ? 0 : (k + 1 - g + G - 1) / G;
-    {
-      const int lane = tid & 63;
-      for (int r = first + (tid >> 6); r < rpw; r += kWaves) {  // one wave per row, no block barrier
-        const int i = g + r * G;
-        if (i >= n) break;
-        const Z* row = rows + size_t(r) * n + (k + 1);
-        Z acc{0., 0.};
-        for (int j = lane; j < m; j += 64) acc = zadd(acc, zmul(row[j], v[j]));
-        acc.x = wave_sum(acc.x);
-        acc.y = wave_sum(acc.y);
-        if (lane == 0) {
-          stage_p[r - first] = acc;
-          stage_c[r - first] = row[0];
-          stage_i[r - first] = i;
-        }
-      }
-      if (tid == 0) {
-        int cnt = 0;
-        for (int r = first; r < rpw && g + r * G < n; ++r) ++cnt;
-        staged = cnt;
-      }
-    }
-    // ---- 3. publication and grid barrier
-    __syncthreads();
+    // ---- 1. publication and grid barrier
     if (!publish_and_wait(flags, abort_flag, unsigned(k + 1), staged, stage_i, stage_p, stage_c, pp, pc)) return;
-    // ---- 4. w, rank-2 update of own rows, next column for all rows; all
-    // loads are issued before the first is used (one round trip)
-    {
-      Z pw[kPerThread], pcv[kPerThread];
+    // ---- 2. all loads issued at once (one round trip); partial (tau p)^H v
+    Z pw[kPerThread], pcv[kPerThread];
+    const Z p0 = load_published(pp + k + 1);
 #pragma unroll
-      for (int q = 0; q < kPerThread; ++q) {
-        const int j = tid + q * kThreads;
-        if (j < m) {
-          pw[q] = load_published(pp + k + 1 + j);
-          pcv[q] = load_published(pc + k + 1 + j);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < kPerThread; ++q) {
-        const int j = tid + q * kThreads;
-        if (j < m) {
-          w[j] = pw[q];
-          col[k + 1 + j] = pcv[q];
-        }
+    for (int q = 0; q < kPerThread; ++q) {
+      const int j = tid + q * kThreads;
+      if (j < m) {
+        pw[q] = load_published(pp + k + 1 + j);
+        pcv[q] = load_published(pc + k + 1 + j);
       }
     }
-    __syncthreads();
-    Z dot{0., 0.};  // (tau p)^H v
-    for (int j = tid; j < m; j += kThreads) dot = zadd(dot, zmul(zconj(zmul(t, w[j])), v[j]));
+    Z dot{0., 0.};
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+      const int j = tid + q * kThreads;
+      if (j < m) dot = zadd(dot, zmul(zconj(zmul(t, pw[q])), v[j]));
+    }
     dot = block_zsum(dot, scratch);
-    if (tid == 0) sh_a2 = zscale(zmul(t, dot), -0.5);
-    __syncthreads();
-    const Z a2 = sh_a2;
-    for (int j = tid; j < m; j += kThreads) w[j] = zadd(zmul(t, w[j]), zmul(a2, v[j]));
-    __syncthreads();
-    for (int r = 0; r < rpw; ++r) {
+    const Z a2 = zscale(zmul(t, dot), -0.5);
+    // ---- 3. w = tau p + a2 v (registers); next column for all rows; rank-2
+    // update of the own rows; |col|^2 of the next reflector's tail
+    const Z w0c = zconj(zadd(zmul(t, p0), a2));  // v[0] = 1
+    double part = 0.;
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+      const int j = tid + q * kThreads;
+      if (j < m) {
+        const Z vj = v[j];
+        const Z wj = zadd(zmul(t, pw[q]), zmul(a2, vj));
+        const Z c = zsub(pcv[q], zadd(zmul(vj, w0c), wj));
+        col[k + 1 + j] = c;
+        if (j >= 2) part += c.x * c.x + c.y * c.y;
+        pw[q] = zconj(wj);   // conj(w_j)
+        pcv[q] = zconj(vj);  // conj(v_j)
+      }
+    }
+    for (int r = first; r < rpw; ++r) {
       const int i = g + r * G;
       if (i >= n) break;
-      if (i <= k) continue;
-      const Z vi = v[i - k - 1], wi = w[i - k - 1];
+      const Z vi = v[i - k - 1];
+      const Z wi = zadd(zmul(t, stage_p[r - first]), zmul(a2, vi));
       Z* row = rows + size_t(r) * n + (k + 1);
-      for (int j = tid; j < m; j += kThreads)
-        row[j] = zsub(row[j], zadd(zmul(vi, zconj(w[j])), zmul(wi, zconj(v[j]))));
+#pragma unroll
+      for (int q = 0; q < kPerThread; ++q) {
+        const int j = tid + q * kThreads;
+        if (j < m) row[j] = zsub(row[j], zadd(zmul(vi, pw[q]), zmul(wi, pcv[q])));
+      }
     }
-    const Z w0c = zconj(w[0]), v0c = zconj(v[0]);
-    for (int j = tid; j < m; j += kThreads)
-      col[k + 1 + j] = zsub(col[k + 1 + j], zadd(zmul(v[j], w0c), zmul(w[j], v0c)));
-    __syncthreads();
+    const double xn2 = block_sum(part, scratch);  // also publishes col and the rows to the block
+    if (k + 2 < n) {
+      // ---- 4. next reflector and the products of step k + 1
+      t = form_reflector(col, k + 1, n, xn2, vn, refl, ldr, d, e, tau);
+      stage_products(rows, n, rpw, k + 1, vn, stage_p, stage_c, stage_i, &staged);
+    } else if (g == 0 && tid == 0) {
+      d[n - 1] = col[n - 1].x;
+    }
   }
-  if (g == 0 && tid == 0) d[n - 1] = col[n - 1].x;
 }
 
 int max_lds() {

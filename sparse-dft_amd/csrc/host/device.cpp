@@ -836,7 +836,7 @@ std::vector<std::complex<double>> DeviceOperator::density(const double* x, long 
     if (len > 128) throw std::invalid_argument("HIP: density grid beyond the device FFT lengths");
     shape[a] = len;
   }
-  if ((std::size_t(shape[0]) * shape[1] + shape[0] + shape[1]) * 16 > std::size_t(max_lds_bytes()))
+  if ((std::size_t(shape[0]) * (shape[1] | 1) + shape[0] + shape[1]) * 16 > std::size_t(max_lds_bytes()))
     throw std::invalid_argument("HIP: density grid plane exceeds the LDS of one workgroup");
   const long count = long(shape[0]) * shape[1] * shape[2];
   std::vector<std::uint64_t> idx(n_);
