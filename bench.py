@@ -115,6 +115,9 @@ def main():
     ap.add_argument("--subspace-factor", type=int, default=6,
                     help="Davidson capacity factor (the reference's SDFT_DAVIDSON_SUBSPACE_FACTOR, 4..16)")
     ap.add_argument("--stagnation-window", type=int, default=2)
+    ap.add_argument("--handoff-restart", type=int, default=1,
+                    help="1: thick-restart to the 2 x roots Ritz vectors at the FP32 -> FP64 hand-off; "
+                         "0: refresh the whole subspace in FP64")
     ap.add_argument("--serving", type=int, default=0,
                     help="also run N serving requests (Si8..Si64 mix, Poisson arrivals) and report their p50/p99")
     ap.add_argument("--serving-workers", type=int, default=4)
@@ -147,7 +150,7 @@ def main():
         t0 = time.perf_counter()
         r = op.davidson(ROOTS, 256, 1e-12, init, search_precision=precision, verbose=args.verbose,
                         timing=args.timing, subspace_factor=args.subspace_factor,
-                        stagnation_window=args.stagnation_window, handoff_restart=1)
+                        stagnation_window=args.stagnation_window, handoff_restart=args.handoff_restart)
         ctx.synchronize()
         r["wall"] = time.perf_counter() - t0
         if not r["converged"]:

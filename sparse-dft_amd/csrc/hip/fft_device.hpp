@@ -20,8 +20,11 @@
 
 namespace sdft_hip {
 
+// Aligned to its full size so that LDS and global accesses of one element are
+// single 128-bit (FP64) or 64-bit (FP32) operations: split into two halves, a
+// wave's consecutive elements would hit every LDS bank twice.
 template <typename T>
-struct cx {
+struct alignas(2 * sizeof(T)) cx {
   T x, y;
 };
 

@@ -165,20 +165,22 @@ size_t plane_lds_bytes(int n0, int n1) {
   return (size_t(n0) * line_stride(n1) + n0 + n1) * sizeof(cx<T>);
 }
 
-// Threads per plane workgroup (one xy-plane per workgroup): 512 unless
-// SDFT_HIP_PLANE_THREADS selects 256 or 1024.
-int plane_threads() {
-  static const int threads = [] {
+// Threads per plane workgroup (one xy-plane per workgroup): 1024 for FP64
+// (one workgroup per CU, so more waves hide more latency) and 512 for FP32
+// (several workgroups per CU), measured; SDFT_HIP_PLANE_THREADS (256, 512 or
+// 1024) overrides both.
+int plane_threads(bool fp64) {
+  static const int forced = [] {
     const char* e = std::getenv("SDFT_HIP_PLANE_THREADS");
-    const int t = e ? std::atoi(e) : 512;
-    return t == 256 || t == 1024 ? t : 512;
+    const int t = e ? std::atoi(e) : 0;
+    return t == 256 || t == 512 || t == 1024 ? t : 0;
   }();
-  return threads;
+  return forced ? forced : (fp64 ? 1024 : 512);
 }
 
 template <typename T, typename... Args>
 void launch_k_planes(dim3 grid, size_t lds, hipStream_t stream, Args... args) {
-  switch (plane_threads()) {
+  switch (plane_threads(sizeof(T) == 8)) {
     case 256:
       hipLaunchKernelGGL((k_planes<T, 256>), grid, dim3(256), lds, stream, args...);
       break;

@@ -33,12 +33,16 @@
 namespace sdft_hip {
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;
 constexpr int kWaves = kThreads / 64;
 constexpr long kSpinLimit = 1L << 20;
 constexpr int kPerThread = (kHetrdMaxOrder + kThreads - 1) / kThreads;  // trailing entries per thread
 
+#ifdef SDFT_PROBE_Z_ALIGN
+struct alignas(16) Z {
+#else
 struct Z {
+#endif
   double x, y;
 };
 __device__ inline Z zmul(Z a, Z b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
@@ -47,11 +51,38 @@ __device__ inline Z zadd(Z a, Z b) { return {a.x + b.x, a.y + b.y}; }
 __device__ inline Z zsub(Z a, Z b) { return {a.x - b.x, a.y - b.y}; }
 __device__ inline Z zscale(Z a, double s) { return {a.x * s, a.y * s}; }
 
+#ifdef SDFT_PROBE_DPP
+// v from lane (lane ^ 1), (lane ^ 2) (quad permutations), the mirrored lane of
+// the 8-lane half row, the mirrored lane of the 16-lane row (DPP controls)
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
+}
+__device__ inline double read_lane(double v, int lane) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane(int(b), lane);
+  const int hi = __builtin_amdgcn_readlane(int(b >> 32), lane);
+  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
+}
+// Sum over the 64 lanes in a fixed order, every lane gets it: pairs, quads,
+// half rows and rows through DPP moves, then the four row sums.
+__device__ inline double wave_sum(double v) {
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  v += dpp_move<0x140>(v);  // row_mirror
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+#else
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+#endif
 
 // Sum of one double over the workgroup in a fixed order; every thread gets it.
 __device__ double block_sum(double v, double* scratch) {
@@ -204,7 +235,11 @@ __global__ __launch_bounds__(kThreads) void k_hetrd_lds(const Z* __restrict__ a,
                                                         double* __restrict__ e, Z* __restrict__ tau,
                                                         Z* __restrict__ pub_p, Z* __restrict__ pub_c,
                                                         unsigned* __restrict__ flags, int* __restrict__ abort_flag) {
+#ifdef SDFT_PROBE_Z_ALIGN
+  extern __shared__ __attribute__((aligned(16))) double lds_raw[];
+#else
   extern __shared__ double lds_raw[];
+#endif
   Z* rows = reinterpret_cast<Z*>(lds_raw);  // rpw x n
   Z* vbuf = rows + size_t(rpw) * n;         // two reflectors (steps of both parities), n each
   Z* col = vbuf + 2 * size_t(n);            // current column, n
