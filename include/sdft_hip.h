@@ -154,6 +154,9 @@ typedef struct {
   int search_precision;
   double search_handoff;
   int stagnation_window;
+  /* FP32 search hand-off: h >= 1 keeps the max(2, h) x roots lowest Ritz
+   * vectors (FP64 images built for those only); 0 re-applies FP64 to the whole
+   * subspace. */
   int handoff_restart;
   int verbose;
   int timing;

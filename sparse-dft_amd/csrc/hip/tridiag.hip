@@ -28,6 +28,9 @@
 // invalidate, which an agent-scope fence would issue on every column.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
+
 #include "tridiag.hpp"
 
 namespace sdft_hip {
@@ -38,11 +41,9 @@ constexpr int kWaves = kThreads / 64;
 constexpr long kSpinLimit = 1L << 20;
 constexpr int kPerThread = (kHetrdMaxOrder + kThreads - 1) / kThreads;  // trailing entries per thread
 
-#ifdef SDFT_PROBE_Z_ALIGN
+// Aligned to 16 bytes: one element is one 128-bit LDS access (the dynamic LDS
+// base is declared 16-byte aligned below, past the static variables).
 struct alignas(16) Z {
-#else
-struct Z {
-#endif
   double x, y;
 };
 __device__ inline Z zmul(Z a, Z b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
@@ -51,7 +52,6 @@ __device__ inline Z zadd(Z a, Z b) { return {a.x + b.x, a.y + b.y}; }
 __device__ inline Z zsub(Z a, Z b) { return {a.x - b.x, a.y - b.y}; }
 __device__ inline Z zscale(Z a, double s) { return {a.x * s, a.y * s}; }
 
-#ifdef SDFT_PROBE_DPP
 // v from lane (lane ^ 1), (lane ^ 2) (quad permutations), the mirrored lane of
 // the 8-lane half row, the mirrored lane of the 16-lane row (DPP controls)
 template <int CTRL>
@@ -68,7 +68,8 @@ __device__ inline double read_lane(double v, int lane) {
   return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
 }
 // Sum over the 64 lanes in a fixed order, every lane gets it: pairs, quads,
-// half rows and rows through DPP moves, then the four row sums.
+// half rows and rows through DPP moves, then the four row sums. Every call
+// site runs it with all 64 lanes active (wave-uniform control flow).
 __device__ inline double wave_sum(double v) {
   v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
   v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
@@ -76,13 +77,7 @@ __device__ inline double wave_sum(double v) {
   v += dpp_move<0x140>(v);  // row_mirror
   return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
-#else
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-#endif
+
 
 // Sum of one double over the workgroup in a fixed order; every thread gets it.
 __device__ double block_sum(double v, double* scratch) {
@@ -235,11 +230,7 @@ __global__ __launch_bounds__(kThreads) void k_hetrd_lds(const Z* __restrict__ a,
                                                         double* __restrict__ e, Z* __restrict__ tau,
                                                         Z* __restrict__ pub_p, Z* __restrict__ pub_c,
                                                         unsigned* __restrict__ flags, int* __restrict__ abort_flag) {
-#ifdef SDFT_PROBE_Z_ALIGN
   extern __shared__ __attribute__((aligned(16))) double lds_raw[];
-#else
-  extern __shared__ double lds_raw[];
-#endif
   Z* rows = reinterpret_cast<Z*>(lds_raw);  // rpw x n
   Z* vbuf = rows + size_t(rpw) * n;         // two reflectors (steps of both parities), n each
   Z* col = vbuf + 2 * size_t(n);            // current column, n
@@ -361,6 +352,19 @@ size_t lds_bytes(int n, int rpw) {
 }
 
 
+// Target rows per workgroup: 4 (measured at n = 320 / 640 / 960 against 6, 10,
+// 16 and as many as the LDS holds: 5.2 / 5.8 / 6.4 us per column, the largest
+// slabs 8.0 / 6.7 / 6.4); SDFT_HIP_HETRD_ROWS overrides. More rows are used
+// when the workgroups would outnumber the CUs.
+int target_rows_per_group() {
+  static const int rows = [] {
+    const char* e = std::getenv("SDFT_HIP_HETRD_ROWS");
+    const int v = e ? std::atoi(e) : 0;
+    return v > 0 ? v : 4;
+  }();
+  return rows;
+}
+
 // LDS the kernel declares statically (the workgroup reductions of the
 // runtime included), which the dynamic slab must leave room for.
 size_t static_lds() {
@@ -379,10 +383,10 @@ bool hetrd_plan(int n, int* rows_per_group, int* groups) {
   const size_t total = size_t(max_lds()), fixed = static_lds();
   if (fixed >= total) return false;
   const size_t budget = total - fixed;
-  // as many rows per workgroup as the LDS holds (fewer workgroups make a
-  // cheaper barrier), one workgroup per CU
-  int rpw = 1;
-  while (lds_bytes(n, rpw + 1) <= budget && rpw < n) ++rpw;
+  // the target rows per workgroup, or more so that there is at most one
+  // workgroup per CU
+  int rpw = std::min(n, target_rows_per_group());
+  if (cus > 0) rpw = std::max(rpw, (n + cus - 1) / cus);
   if (lds_bytes(n, rpw) > budget) return false;
   const int g = (n + rpw - 1) / rpw;
   if (cus > 0 && g > cus) return false;

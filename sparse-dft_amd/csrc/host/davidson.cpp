@@ -666,7 +666,10 @@ DavidsonOutput Solver::run(const C* initial) {
     double* B = rrb.p;
     check(launch_hermitise(ch_.p, cap_, size_, A, size_, s_), "hermitise");
     check(launch_hermitise(cs_.p, cap_, size_, B, size_, s_), "hermitise");
-    const std::vector<double> w = rr_solve(A, B, size_, std::min(size_, 2 * R));
+    // Ritz vectors wanted: 2 roots for a thick restart, more during an FP32
+    // search whose hand-off keeps more
+    const long wanted = exact_ ? 2 * R : long(std::max(2, p_.handoff_restart)) * R;
+    const std::vector<double> w = rr_solve(A, B, size_, std::min(size_, wanted));
     for (long r = 0; r < R; ++r) values_[r] = w[r];
     // Ritz vectors and residuals (src/pw_core.cpp:1169-1183)
     gemm(exact_, 0, 0, n, R, size_, 1.0, basis_.p, n, A, size_, x_.p, n);
@@ -688,10 +691,10 @@ DavidsonOutput Solver::run(const C* initial) {
       const bool stalled = since_halving >= p_.stagnation_window;
       if (max_res() < p_.search_handoff || it + 1 == p_.max_iterations || stalled) {
         stats_.handoff_iteration = it + 1;
-        if (p_.handoff_restart && size_ > 2 * R) {
-          // keep the 2 roots lowest Ritz vectors (FP64 rotation), then the
-          // exact block CGS2 and the next flush give them FP64 S and H images
-          const long keep = std::min(size_, 2 * R);
+        const long keep = std::min(size_, long(std::max(2, p_.handoff_restart)) * R);
+        if (p_.handoff_restart && size_ > keep) {
+          // keep the lowest Ritz vectors (FP64 rotation), then the exact block
+          // CGS2 and the next flush give them FP64 S and H images
           exact_ = true;
           gemm(true, 0, 0, n, keep, size_, 1.0, basis_.p, n, A, size_, retained.p, n);
           size_ = 0;

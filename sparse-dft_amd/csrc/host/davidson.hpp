@@ -20,8 +20,9 @@ struct DavidsonParams {
   int search_fp32 = 0;            // 1: FP32 H/S applies and subspace GEMMs until hand-off
   double search_handoff = 1e-8;   // max residual that hands the search to FP64 (src/pw_core.cpp:1184)
   int stagnation_window = 3;      // FP32 only: hand off when max residual stops halving over this many iterations
-  int handoff_restart = 1;        // FP32 only: thick-restart to 2 roots Ritz vectors at the hand-off and build
-                                  // FP64 images for those, instead of re-applying FP64 to the whole subspace
+  int handoff_restart = 1;        // FP32 only: h >= 1 thick-restarts to max(2, h) x roots Ritz vectors at the
+                                  // hand-off and builds FP64 images for those; 0 re-applies FP64 to the whole
+                                  // subspace
   int verbose = 0;
   int timing = 0;                 // synchronise phases and accumulate per-phase times
   long band_group_size = 0;       // > 0: bounded band-group solver (src/band_group_solver.cpp:57-211)
