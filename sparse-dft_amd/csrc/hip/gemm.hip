@@ -69,6 +69,12 @@ __device__ __forceinline__ TileId xcd_tile(int mt, int nt, int splits) {
   return {(t / nt) % mt, t % nt, t / (nt * mt)};
 }
 
+// 1: FP64 complex products as three real MFMA products (Gauss / 3M) instead of
+// four; FP32 keeps four.
+#ifndef SDFT_GEMM_GAUSS_FP64
+#define SDFT_GEMM_GAUSS_FP64 0
+#endif
+
 // At least SDFT_GEMM_MIN_WAVES waves per SIMD: left alone, the FP64 tiles keep
 // their accumulators in AGPRs and the alpha/beta epilogue copies them all to
 // VGPRs at once, which caps occupancy at two waves.
@@ -91,7 +97,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDFT_GEMM_M
   const int m0 = tile.m * TBM, n0 = tile.n * TBN;
   const int k0 = tile.split * kps, k1 = min(K, k0 + kps);
   const int wm = (wave % WGM) * WM, wn = (wave / WGM) * WN;
-  V acc_re[IM][JN], acc_im[IM][JN];
+  // FP64 with kGauss: three real products per block (Gauss / 3M),
+  // acc_re = sum Ar Br, acc_x = sum Ai Bi, acc_im = sum (Ar + Ai)(Br + Bi)
+  constexpr bool kGauss = sizeof(T) == 8 && SDFT_GEMM_GAUSS_FP64;
+  constexpr int GI = kGauss ? IM : 1, GJ = kGauss ? JN : 1;
+  V acc_re[IM][JN], acc_im[IM][JN], acc_x[GI][GJ];
 #pragma unroll
   for (int i = 0; i < IM; ++i)
 #pragma unroll
@@ -99,6 +109,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDFT_GEMM_M
       acc_re[i][j] = V{0, 0, 0, 0};
       acc_im[i][j] = V{0, 0, 0, 0};
     }
+#pragma unroll
+  for (int i = 0; i < GI; ++i)
+#pragma unroll
+    for (int j = 0; j < GJ; ++j) acc_x[i][j] = V{0, 0, 0, 0};
   double pa[EA][2], pb[EB][2];
   // A_eff[m][k] and B_eff[k][n] of the K slab at kb into registers
   auto load = [&](int kb) {
@@ -169,20 +183,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDFT_GEMM_M
         br[j] = bs_re[kr][wn + 16 * j + c];
         bi[j] = bs_im[kr][wn + 16 * j + c];
       }
+      if constexpr (kGauss) {
+        T as[IM], bsum[JN];
 #pragma unroll
-      for (int i = 0; i < IM; ++i)
+        for (int i = 0; i < IM; ++i) as[i] = ar[i] + ai[i];
 #pragma unroll
-        for (int j = 0; j < JN; ++j) {
-          acc_re[i][j] = Acc<T>::mfma(ar[i], br[j], acc_re[i][j]);
-          acc_im[i][j] = Acc<T>::mfma(ar[i], bi[j], acc_im[i][j]);
-        }
+        for (int j = 0; j < JN; ++j) bsum[j] = br[j] + bi[j];
 #pragma unroll
-      for (int i = 0; i < IM; ++i)
+        for (int i = 0; i < IM; ++i)
 #pragma unroll
-        for (int j = 0; j < JN; ++j) {
-          acc_re[i][j] = Acc<T>::mfma(-ai[i], bi[j], acc_re[i][j]);
-          acc_im[i][j] = Acc<T>::mfma(ai[i], br[j], acc_im[i][j]);
-        }
+          for (int j = 0; j < JN; ++j) {
+            acc_re[i][j] = Acc<T>::mfma(ar[i], br[j], acc_re[i][j]);
+            acc_im[i][j] = Acc<T>::mfma(as[i], bsum[j], acc_im[i][j]);
+          }
+#pragma unroll
+        for (int i = 0; i < GI; ++i)
+#pragma unroll
+          for (int j = 0; j < GJ; ++j) acc_x[i][j] = Acc<T>::mfma(ai[i], bi[j], acc_x[i][j]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < IM; ++i)
+#pragma unroll
+          for (int j = 0; j < JN; ++j) {
+            acc_re[i][j] = Acc<T>::mfma(ar[i], br[j], acc_re[i][j]);
+            acc_im[i][j] = Acc<T>::mfma(ar[i], bi[j], acc_im[i][j]);
+          }
+#pragma unroll
+        for (int i = 0; i < IM; ++i)
+#pragma unroll
+          for (int j = 0; j < JN; ++j) {
+            acc_re[i][j] = Acc<T>::mfma(-ai[i], bi[j], acc_re[i][j]);
+            acc_im[i][j] = Acc<T>::mfma(ai[i], br[j], acc_im[i][j]);
+          }
+      }
     }
     __syncthreads();
   }
@@ -200,7 +233,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDFT_GEMM_M
         const int m = m0 + wm + 16 * i + row;
         const int n = n0 + wn + 16 * j + (lane & 15);
         if (m >= M || n >= N) continue;
-        double re = alpha * double(acc_re[i][j][r]), im = alpha * double(acc_im[i][j][r]);
+        double re = double(acc_re[i][j][r]), im = double(acc_im[i][j][r]);
+        if constexpr (kGauss) {
+          const double s2 = double(acc_x[kGauss ? i : 0][kGauss ? j : 0][r]);
+          im = im - re - s2;
+          re = re - s2;
+        }
+        re *= alpha;
+        im *= alpha;
         if (split) {
           double* w = work + 2 * ((long)tile.split * M * N + m + (long)n * M);
           w[0] = re;
