@@ -196,12 +196,20 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
     peak, unit, bound, _ = peaks[dom]
     traffic = measured_traffic().get(dom)
+    # the GEMM kinds count 8 m n k flops; the FP64 kernels execute 2 x 3 m n k
+    # real MFMA flops when they use the Gauss (3M) form
+    products = {"gemm_fp64": native.gemm_real_products(0), "gemm_fp32": native.gemm_real_products(1)}
+    for k in kernels:
+        if k in products:
+            kernels[k]["real_products_per_complex_mac"] = products[k]
+            kernels[k]["mfma_executed_frac"] = kernels[k]["frac"] * products[k] / 4
     roofline = {"bound": bound, "achieved": kernels[dom]["achieved"], "peak": peak, "unit": unit,
                 "frac": kernels[dom]["frac"], "traffic": traffic, "kernel": names[dom],
                 "work_per_launch": kernels[dom]["work_per_launch"],
                 "work_unit": "flops (8 m n k)" if bound == "mfma" else "algorithmic bytes",
                 "ms_per_launch": kernels[dom]["ms_per_launch"], "launches": kernels[dom]["launches"],
                 "algorithmic_bytes_per_launch": kernels[dom]["algorithmic_bytes_per_launch"],
+                "mfma_executed_frac": kernels[dom].get("mfma_executed_frac"),
                 "traffic_source": "profiles/si64_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
                 if traffic is not None else None,
                 "measured": "HIP events on the launch stream over the timed steps"}

@@ -58,6 +58,10 @@ int sdft_hip_synchronize(sdft_hip_context* ctx);
 #define SDFT_HIP_CLOCK_LOCAL_FP32 3
 int sdft_hip_kernel_clock(sdft_hip_context* ctx, int enable);
 int sdft_hip_kernel_clock_stats(sdft_hip_context* ctx, int kind, long* launches, double* milliseconds, double* work);
+/* Real MFMA products per complex multiply-add in the complex GEMM kernels of
+ * `precision` (0 FP64, 1 FP32): 3 for the Gauss form, 4 otherwise. Measurement
+ * only (the conventional complex GEMM flop count 8 m n k corresponds to 4). */
+int sdft_hip_gemm_real_products(int precision, int* products);
 /* Algorithmic HBM bytes summed over the same launches (GEMM: operands and
  * result once each; local H: the per-launch byte count above). */
 int sdft_hip_kernel_clock_bytes(sdft_hip_context* ctx, int kind, double* bytes);

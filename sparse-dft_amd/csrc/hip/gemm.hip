@@ -69,20 +69,19 @@ __device__ __forceinline__ TileId xcd_tile(int mt, int nt, int splits) {
   return {(t / nt) % mt, t % nt, t / (nt * mt)};
 }
 
-// 1: FP64 complex products as three real MFMA products (Gauss / 3M) instead of
-// four; FP32 keeps four.
+// FP64 complex products as three real MFMA products (Gauss / 3M: Ar Br, Ai Bi,
+// (Ar + Ai)(Br + Bi)) instead of four, 25 % fewer MFMA cycles; FP32 keeps four.
+// SDFT_GEMM_GAUSS_FP64=0 builds the four-product form.
 #ifndef SDFT_GEMM_GAUSS_FP64
-#define SDFT_GEMM_GAUSS_FP64 0
+#define SDFT_GEMM_GAUSS_FP64 1
 #endif
 
-// At least SDFT_GEMM_MIN_WAVES waves per SIMD: left alone, the FP64 tiles keep
+// WAVES: at least that many waves per SIMD. Left alone, the FP64 tiles keep
 // their accumulators in AGPRs and the alpha/beta epilogue copies them all to
-// VGPRs at once, which caps occupancy at two waves.
-#ifndef SDFT_GEMM_MIN_WAVES
-#define SDFT_GEMM_MIN_WAVES 3
-#endif
-template <typename T, typename TA, typename TB, bool ADJ_A, bool ADJ_B, int TBM, int TBN, int WGM, bool PF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDFT_GEMM_MIN_WAVES))) void k_zgemm(int M, int N, int K, const TA* __restrict__ A, long lda,
+// VGPRs at once, which caps occupancy at two waves; the 3M tiles hold a third
+// accumulator set, which fits three waves only in the split-K plan.
+template <typename T, typename TA, typename TB, bool ADJ_A, bool ADJ_B, int TBM, int TBN, int WGM, bool PF, int WAVES>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_zgemm(int M, int N, int K, const TA* __restrict__ A, long lda,
                                                const TB* __restrict__ B, long ldb, double* __restrict__ C, long ldc,
                                                const double* __restrict__ D, long ldd, double alpha, double beta,
                                                int kps, double* __restrict__ work) {
@@ -376,9 +375,12 @@ hipError_t launch_cfg(int aop, int bop, dim3 grid, int M, int N, int K, const TA
                       double* work, hipStream_t s) {
   constexpr int TBM = kTiles[CFG].bm, TBN = kTiles[CFG].bn, WGM = CFG == 2 ? 4 : 2;
   constexpr bool PF = CFG != 1;
+  // measured per plan on the Si64 shapes: 3M FP64 split-K slabs fastest at
+  // three waves, the prefetching plans at two (three spill there)
+  constexpr int WAVES = (sizeof(T) == 8 && SDFT_GEMM_GAUSS_FP64 && CFG != 1) ? 2 : 3;
 #define SDFT_LAUNCH(AA, BB)                                                                                      \
-  hipLaunchKernelGGL((k_zgemm<T, TA, TB, AA, BB, TBM, TBN, WGM, PF>), grid, dim3(256), 0, s, M, N, K, A, lda, B, \
-                     ldb, C, ldc, D, ldd, alpha, beta, kps, work)
+  hipLaunchKernelGGL((k_zgemm<T, TA, TB, AA, BB, TBM, TBN, WGM, PF, WAVES>), grid, dim3(256), 0, s, M, N, K, A, \
+                     lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work)
   if (!aop && !bop) SDFT_LAUNCH(false, false);
   else if (aop && !bop) SDFT_LAUNCH(true, false);
   else if (!aop && bop) SDFT_LAUNCH(false, true);
@@ -423,6 +425,8 @@ Plan plan_for(bool fp64, int M, int N, int K, bool have_work) {
   return p;
 }
 }  // namespace
+
+int zgemm_real_products(Precision prec) { return prec == Precision::FP64 && SDFT_GEMM_GAUSS_FP64 ? 3 : 4; }
 
 std::size_t zgemm_workspace_bytes(int M, int N, int K) {
   const int splits = zgemm_splits(M, N, K);

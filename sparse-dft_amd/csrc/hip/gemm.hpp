@@ -23,6 +23,9 @@ std::size_t zgemm_workspace_bytes(int M, int N, int K);
 hipError_t zgemm(Precision prec, int aop, int bop, int M, int N, int K, double alpha, const double* A, long lda,
                  const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta, double* work,
                  std::size_t work_bytes, hipStream_t s, const LaunchEvents* ev = nullptr);
+// Real MFMA products per complex multiply-add of the kernels: 3 (Gauss form)
+// or 4. The conventional 8 m n k flop count of a complex GEMM corresponds to 4.
+int zgemm_real_products(Precision prec);
 // Same with A stored in FP32 (resident search copy of the projectors).
 hipError_t zgemm_f32_storage(int aop, int bop, int M, int N, int K, double alpha, const float* A, long lda,
                              const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta,

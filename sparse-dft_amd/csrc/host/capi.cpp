@@ -88,6 +88,13 @@ int sdft_hip_capabilities(char* buffer, size_t size) {
   });
 }
 
+int sdft_hip_gemm_real_products(int precision, int* products) {
+  return guarded([&] {
+    if (!products) throw std::invalid_argument("HIP: null output");
+    *products = sdft_hip::zgemm_real_products(to_precision(precision));
+  });
+}
+
 int sdft_hip_context_create(int device, sdft_hip_context** out) {
   return guarded([&] {
     if (!out) throw std::invalid_argument("HIP: null context output");

@@ -83,6 +83,13 @@ def _check(rc):
         raise HipError(msg)
 
 
+def gemm_real_products(precision=0) -> int:
+    """Real MFMA products per complex multiply-add of the GEMM kernels (3 or 4)."""
+    v = ct.c_int()
+    _check(lib().sdft_hip_gemm_real_products(ct.c_int(precision), ct.byref(v)))
+    return v.value
+
+
 def capabilities() -> str:
     buf = ct.create_string_buffer(1024)
     _check(lib().sdft_hip_capabilities(buf, ct.c_size_t(1024)))
