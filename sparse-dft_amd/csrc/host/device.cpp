@@ -169,12 +169,29 @@ Context::Context(int device) : device_(device) {
     throw std::runtime_error("HIP: rocblas_create_handle failed");
   }
   check(rocblas_set_stream(blas_, stream_), "rocblas_set_stream");
+  check(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking), "hipStreamCreate");
+  check(hipEventCreateWithFlags(&fork_event_, hipEventDisableTiming), "hipEventCreate");
+  check(hipEventCreateWithFlags(&join_event_, hipEventDisableTiming), "hipEventCreate");
+}
+
+void Context::fork() {
+  check(hipEventRecord(fork_event_, stream_), "fork record");
+  check(hipStreamWaitEvent(aux_, fork_event_, 0), "fork wait");
+}
+
+void Context::join() {
+  check(hipEventRecord(join_event_, aux_), "join record");
+  check(hipStreamWaitEvent(stream_, join_event_, 0), "join wait");
 }
 
 Context::~Context() {
+  if (aux_) (void)hipStreamSynchronize(aux_);
   (void)hipStreamSynchronize(stream_);
   for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+  if (fork_event_) (void)hipEventDestroy(fork_event_);
+  if (join_event_) (void)hipEventDestroy(join_event_);
   if (blas_) rocblas_destroy_handle(blas_);
+  if (aux_) (void)hipStreamDestroy(aux_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -904,6 +921,26 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
   hipStream_t s = ctx_.stream();
   const long n = long(n_), np = long(np_);
   const bool want_h = which != 1, want_s = which != 0;
+  // The local Hamiltonian (LDS/latency bound FFT pipeline) runs on the
+  // context's second stream beside the projector chain (MFMA GEMMs) on the
+  // main stream; the H back-projection adds to its output after the join.
+  const bool overlap = want_h && np;
+  if (want_h) {
+    hipStream_t ls = overlap ? ctx_.aux_stream() : s;
+    if (overlap) ctx_.fork();
+    const std::size_t bytes = std::size_t(c) * grid_.dev.nsticks * shape_[2] * (prec == Precision::FP64 ? 16 : 8);
+    t1_.reserve(bytes);
+    t2_.reserve(bytes);
+    const bool f64 = prec == Precision::FP64;
+    const double local_bytes = local_algorithmic_bytes(n, count_, c, f64);
+    const LaunchEvents ev = ctx_.clock_launch(f64 ? Context::kLocalFP64 : Context::kLocalFP32, local_bytes, local_bytes);
+    if (ev.begin) check(hipEventRecord(ev.begin, ls), "hipEventRecord");
+    if (f64)
+      check(launch_local_h<double>(grid_.dev, x, n, c, kinetic_d_.p, hy, n, t1_.p, t2_.p, ls), "local H");
+    else
+      check(launch_local_h<float>(grid_.dev, x, n, c, kinetic_d_.p, hy, n, t1_.p, t2_.p, ls), "local H");
+    if (ev.end) check(hipEventRecord(ev.end, ls), "hipEventRecord");
+  }
   if (np && form_ != 3) {
     // PWO2 dense D/Q on P = B X, or PWO1 folded DB/QB on X
     // (src/plane_wave_operator.cpp:290-301)
@@ -937,20 +974,6 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
                              qblk_.p, p_.p, np, w_.p, np, mask, s),
           "onsite blocks");
   }
-  if (want_h) {
-    const std::size_t bytes = std::size_t(c) * grid_.dev.nsticks * shape_[2] * (prec == Precision::FP64 ? 16 : 8);
-    t1_.reserve(bytes);
-    t2_.reserve(bytes);
-    const bool f64 = prec == Precision::FP64;
-    const double local_bytes = local_algorithmic_bytes(n, count_, c, f64);
-    const LaunchEvents ev = ctx_.clock_launch(f64 ? Context::kLocalFP64 : Context::kLocalFP32, local_bytes, local_bytes);
-    if (ev.begin) check(hipEventRecord(ev.begin, s), "hipEventRecord");
-    if (f64)
-      check(launch_local_h<double>(grid_.dev, x, n, c, kinetic_d_.p, hy, n, t1_.p, t2_.p, s), "local H");
-    else
-      check(launch_local_h<float>(grid_.dev, x, n, c, kinetic_d_.p, hy, n, t1_.p, t2_.p, s), "local H");
-    if (ev.end) check(hipEventRecord(ev.end, s), "hipEventRecord");
-  }
   auto backproject = [&](const double* w, double* out, const double* d, double beta) {
     if (prec == Precision::FP64)
       ctx_.zgemm(prec, 1, 0, n, c, np, 1.0, b64_.p, np, w, np, out, n, d, n, beta);
@@ -965,8 +988,11 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
     }
   };
   if (np) {
-    if (want_h) backproject(w_.p, hy, hy, 1.0);
     if (want_s) backproject(w_.p + 2 * np * c, sy, x, 1.0 + reg_);
+    if (want_h) {
+      if (overlap) ctx_.join();
+      backproject(w_.p, hy, hy, 1.0);
+    }
   } else if (want_s) {
     check(launch_scale_copy(x, n, sy, n, n, c, 1.0 + reg_, s), "overlap identity");
   }

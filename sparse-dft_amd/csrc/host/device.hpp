@@ -86,6 +86,13 @@ class Context {
   ~Context();
   int device() const { return device_; }
   hipStream_t stream() const { return stream_; }
+  // Second stream of the context (default priority) for work that overlaps the
+  // main stream inside one operation: fork() makes it wait for everything
+  // queued on the main stream so far, join() makes the main stream wait for
+  // everything queued on it.
+  hipStream_t aux_stream() const { return aux_; }
+  void fork();
+  void join();
   rocblas_handle blas() const { return blas_; }
   const std::string& name() const { return name_; }
   void sync() { check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
@@ -136,6 +143,8 @@ class Context {
  private:
   int device_;
   hipStream_t stream_{};
+  hipStream_t aux_{};
+  hipEvent_t fork_event_{}, join_event_{};
   rocblas_handle blas_{};
   std::string name_;
   DevBuf<double> work_;
