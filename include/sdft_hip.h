@@ -182,6 +182,8 @@ typedef struct {
   double seconds_total, seconds_apply, seconds_gemm, seconds_ritz, seconds_ortho;
   long cholesky_blocks; /* correction blocks normalised by Cholesky QR */
   long rr_cholesky;     /* Rayleigh-Ritz solves that took the Cholesky reduction */
+  long rr_stedc;        /* Rayleigh-Ritz tridiagonal solves by divide and conquer (inverse iteration
+                           failed its checks, or SDFT_HIP_TRIDIAG_EIGEN=stedc) */
 } sdft_hip_davidson_result;
 
 /* PlaneWaveOperator::project_into / projector_metric_into
@@ -250,7 +252,20 @@ int sdft_hip_snapshot_save(sdft_hip_operator* op, const char* path, size_t roots
  * (n x k, may be NULL when k = 0). Orders above the one-launch reduction's
  * limit take rocSOLVER zheevd. */
 int sdft_hip_hermitian_eigen(sdft_hip_context* ctx, size_t n, const double* a, size_t k, double* values,
-                             double* vectors, int* fused /* out, may be NULL: 1 = one-launch tridiagonalisation */);
+                             double* vectors,
+                             int* fused /* out, may be NULL: 1 = one-launch tridiagonalisation and inverse
+                                           iteration, 2 = one-launch tridiagonalisation and divide and
+                                           conquer, 0 = zheevd */);
+
+/* Symmetric tridiagonal stage of that eigensolver (no counterpart in the
+ * reference; exposed for tests): all n eigenvalues of T (diagonal d, off-
+ * diagonal e of length n - 1) by Sturm-count multisection, ascending to within
+ * the bisection tolerance, and the eigenvectors of the lowest k by inverse
+ * iteration into vectors (n x k real, column-major), before the symmetric
+ * orthonormalisation. *status is 0, or 1 when an iteration did not stay
+ * finite. Host pointers; n <= 2048. */
+int sdft_hip_tridiagonal_eigen(sdft_hip_context* ctx, size_t n, const double* d, const double* e, size_t k,
+                               double* values, double* vectors, int* status);
 
 int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* params, const double* initial,
                       sdft_hip_davidson_result* result);

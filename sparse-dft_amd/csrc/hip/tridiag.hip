@@ -174,7 +174,11 @@ __device__ Z form_reflector(const Z* col, int c0, int n, double xn2, Z* v, Z* __
   Z t{0., 0.}, scale{0., 0.};
   double beta;
   if (xn2 == 0. && alpha.y == 0.) {
-    beta = alpha.x;
+    // column already reduced: H = I - 2 e1 e1^T (beta = -alpha) instead of
+    // LAPACK's tau = 0; rocSOLVER zunmtr returned NaN columns for reflector
+    // sets with tau = 0 at orders of a few hundred (real tridiagonal inputs)
+    beta = -alpha.x;
+    t = {2., 0.};
   } else {
     const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
     beta = alpha.x >= 0. ? -nrm : nrm;

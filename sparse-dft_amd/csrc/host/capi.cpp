@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../hip/davidson_kernels.hpp"
+#include "../hip/tridiag_eig.hpp"
 #include "davidson.hpp"
 #include "dense_eigen.hpp"
 #include "device.hpp"
@@ -534,6 +535,7 @@ int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* par
     result->seconds_ortho = out.stats.seconds_ortho;
     result->cholesky_blocks = out.stats.cholesky_blocks;
     result->rr_cholesky = out.stats.rr_cholesky;
+    result->rr_stedc = out.stats.rr_stedc;
   });
 }
 
@@ -678,7 +680,36 @@ int sdft_hip_hermitian_eigen(sdft_hip_context* ctx, size_t n, const double* a, s
     std::copy(w.begin(), w.end(), values);
     if (k) sdft_hip::check(hipMemcpyAsync(vectors, vec.p, 16 * n * k, hipMemcpyDeviceToHost, s), "download");
     cx.sync();
-    if (fused) *fused = eig.launches_fused() > 0 ? 1 : 0;
+    if (fused) *fused = eig.launches_fused() > 0 ? (eig.stedc_solves() > 0 ? 2 : 1) : 0;
+  });
+}
+
+int sdft_hip_tridiagonal_eigen(sdft_hip_context* ctx, size_t n, const double* d, const double* e, size_t k,
+                               double* values, double* vectors, int* status) {
+  return guarded([&] {
+    if (!ctx || !d || (n > 1 && !e) || !values || !status || n == 0 || k > n || (k && !vectors) ||
+        n > size_t(sdft_hip::kTridiagEigMaxOrder))
+      throw std::invalid_argument("HIP: invalid tridiagonal eigensolve arguments");
+    auto& cx = ctx->impl;
+    hipStream_t s = cx.stream();
+    sdft_hip::DevBuf<double> buf, z;
+    buf.reserve(3 * n + 2);
+    double* dd = buf.p;
+    double* de = dd + n;
+    double* dw = de + n;
+    int* st = reinterpret_cast<int*>(dw + n);
+    z.reserve(2 * n * std::max<size_t>(k, 1));
+    sdft_hip::check(hipMemcpyAsync(dd, d, 8 * n, hipMemcpyHostToDevice, s), "upload");
+    if (n > 1) sdft_hip::check(hipMemcpyAsync(de, e, 8 * (n - 1), hipMemcpyHostToDevice, s), "upload");
+    sdft_hip::check(sdft_hip::launch_tridiag_values(int(n), dd, de, dw, st, s), "tridiagonal eigenvalues");
+    sdft_hip::check(sdft_hip::launch_tridiag_vectors(int(n), dd, de, dw, int(k), z.p, long(n), st, s),
+                    "tridiagonal eigenvectors");
+    std::vector<double> zh(2 * n * k);
+    sdft_hip::check(hipMemcpyAsync(values, dw, 8 * n, hipMemcpyDeviceToHost, s), "download");
+    if (k) sdft_hip::check(hipMemcpyAsync(zh.data(), z.p, 16 * n * k, hipMemcpyDeviceToHost, s), "download");
+    sdft_hip::check(hipMemcpyAsync(status, st, sizeof(int), hipMemcpyDeviceToHost, s), "download");
+    cx.sync();
+    for (size_t i = 0; i < n * k; ++i) vectors[i] = zh[2 * i];
   });
 }
 

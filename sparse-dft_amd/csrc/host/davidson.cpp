@@ -811,6 +811,7 @@ DavidsonOutput Solver::run(const C* initial) {
   check(hipMemcpyAsync(out.eigenvectors.data(), x_.p, 16 * std::size_t(n) * R, hipMemcpyDeviceToHost, s_), "vectors");
   sync();
   stats_.iterations = iterations;
+  stats_.rr_stedc = eig_ ? eig_->stedc_solves() : 0;
   stats_.converged = converged_;
   stats_.seconds_total = now() - t_start;
   out.stats = stats_;

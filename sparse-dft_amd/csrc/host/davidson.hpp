@@ -43,6 +43,7 @@ struct DavidsonStats {
   double seconds_total = 0, seconds_apply = 0, seconds_gemm = 0, seconds_ritz = 0, seconds_ortho = 0;
   long cholesky_blocks = 0;  // correction blocks normalised by Cholesky QR
   long rr_cholesky = 0;      // Rayleigh-Ritz solves that needed the Cholesky reduction
+  long rr_stedc = 0;         // tridiagonal solves by divide and conquer instead of inverse iteration
   std::vector<double> max_residual_history;
 };
 

@@ -1,7 +1,8 @@
 // Dense Hermitian eigensolver of the Rayleigh-Ritz step: one-launch Householder
-// tridiagonalisation (tridiag.hip), divide and conquer on the tridiagonal
-// (rocSOLVER zstedc) and the back-transformation of only the wanted columns
-// (rocSOLVER zunmtr).
+// tridiagonalisation (tridiag.hip), the tridiagonal eigenvalues by multisection
+// and the wanted eigenvectors by inverse iteration (tridiag_eig.hip; rocSOLVER
+// zstedc divide and conquer when that fails its checks or is selected), and the
+// back-transformation of only the wanted columns (rocSOLVER zunmtr).
 #pragma once
 #include <vector>
 
@@ -17,13 +18,18 @@ class HermitianEigensolver {
   // (device, ld m).
   std::vector<double> solve(const double* a, long m, long k, double* vecs);
   long launches_fused() const { return fused_; }
+  // inverse-iteration solves that failed their checks and were redone by
+  // divide and conquer; tridiagonal solves by divide and conquer in total
+  long fallbacks() const { return fallbacks_; }
+  long stedc_solves() const { return stedc_; }
 
  private:
   Context& ctx_;
   long max_order_;
-  DevBuf<double> refl_, z_, small_;
+  bool inverse_iteration(long m, long k, double* vecs, std::vector<double>& values);
+  DevBuf<double> refl_, z_, small_, orth_;
   DevBuf<char> work_;
-  long fused_ = 0;
+  long fused_ = 0, fallbacks_ = 0, stedc_ = 0;
 };
 
 }  // namespace sdft_hip

@@ -41,7 +41,7 @@ class DavidsonResult(ct.Structure):
                 ("converged", ct.c_int), ("h_columns_search", ct.c_long), ("h_columns_exact", ct.c_long),
                 ("handoff_iteration", ct.c_long), ("seconds_total", ct.c_double), ("seconds_apply", ct.c_double),
                 ("seconds_gemm", ct.c_double), ("seconds_ritz", ct.c_double), ("seconds_ortho", ct.c_double),
-                ("cholesky_blocks", ct.c_long), ("rr_cholesky", ct.c_long)]
+                ("cholesky_blocks", ct.c_long), ("rr_cholesky", ct.c_long), ("rr_stedc", ct.c_long)]
 
 
 class SnapshotInfo(ct.Structure):
@@ -156,7 +156,9 @@ class Context:
     def hermitian_eigen(self, a, k=None):
         """All eigenvalues (ascending) of the Hermitian matrix a and the lowest k
         eigenvectors, through the device Rayleigh-Ritz eigensolver; returns
-        (values, vectors, fused)."""
+        (values, vectors, path): path 1 = one-launch tridiagonalisation with
+        multisection and inverse iteration, 2 = one-launch tridiagonalisation
+        with divide and conquer, 0 = rocSOLVER zheevd."""
         a = np.asfortranarray(a, dtype=np.complex128)
         n = a.shape[0]
         k = n if k is None else int(k)
@@ -165,7 +167,20 @@ class Context:
         fused = ct.c_int(0)
         _check(lib().sdft_hip_hermitian_eigen(self.handle, ct.c_size_t(n), _p(a), ct.c_size_t(k), _p(w),
                                               _p(v) if k else None, ct.byref(fused)))
-        return w, v, bool(fused.value)
+        return w, v, int(fused.value)
+
+    def tridiagonal_eigen(self, d, e, k):
+        """Device tridiagonal stage: (all eigenvalues, n x k real inverse-iteration
+        vectors before orthonormalisation, status)."""
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        n = d.size
+        w = np.zeros(n)
+        v = np.zeros((n, k), order="F")
+        st = ct.c_int(0)
+        _check(lib().sdft_hip_tridiagonal_eigen(self.handle, ct.c_size_t(n), _p(d), _p(e) if n > 1 else None,
+                                                ct.c_size_t(k), _p(w), _p(v) if k else None, ct.byref(st)))
+        return w, v, st.value
 
     CLOCK_KINDS = {"gemm_fp64": 0, "gemm_fp32": 1, "local_fp64": 2, "local_fp32": 3}
 
@@ -450,4 +465,4 @@ class Operator:
                     h_columns_exact=r.h_columns_exact, handoff_iteration=r.handoff_iteration,
                     seconds_total=r.seconds_total, seconds_apply=r.seconds_apply, seconds_gemm=r.seconds_gemm,
                     seconds_ritz=r.seconds_ritz, seconds_ortho=r.seconds_ortho,
-                    cholesky_blocks=r.cholesky_blocks, rr_cholesky=r.rr_cholesky)
+                    cholesky_blocks=r.cholesky_blocks, rr_cholesky=r.rr_cholesky, rr_stedc=r.rr_stedc)

@@ -104,7 +104,7 @@ def replay(snapshot, report, max_iterations=None, warm_snapshot=None, ctx=None, 
     perf = {"device": ctx.device_name(), "search_precision": search_precision,
             "guard_columns_validated": 0 if guards is None else int(guards.shape[1]),
             **{k: r[k] for k in ("seconds_total", "seconds_apply", "seconds_gemm", "seconds_ritz", "seconds_ortho",
-                                 "h_columns_search", "h_columns_exact", "cholesky_blocks", "rr_cholesky")}}
+                                 "h_columns_search", "h_columns_exact", "cholesky_blocks", "rr_cholesky", "rr_stedc")}}
     with open(report + ".performance.json", "w") as f:
         f.write(json.dumps(perf) + "\n")
     if warm_snapshot:

@@ -1,7 +1,8 @@
 """GPU parity of the Rayleigh-Ritz eigensolver (one-launch Householder
-tridiagonalisation + divide and conquer + back-transformation) against
-LAPACK (numpy.linalg.eigh), on random, clustered, degenerate and structured
-Hermitian matrices, at the orders the block Davidson uses."""
+tridiagonalisation, tridiagonal eigenvalues by multisection and eigenvectors
+by inverse iteration, back-transformation) against LAPACK (numpy.linalg.eigh),
+on random, clustered, degenerate and structured Hermitian matrices, at the
+orders the block Davidson uses."""
 import numpy as np
 import pytest
 
@@ -28,7 +29,7 @@ def test_random_hermitian(hip_ctx, n):
     a = random_hermitian(n, n)
     k = min(n, 320)
     w, v, fused = hip_ctx.hermitian_eigen(a, k)
-    assert fused
+    assert fused == 1
     check_pairs(a, w, v, k, np.linalg.eigvalsh(a))
 
 
@@ -43,13 +44,63 @@ def test_degenerate_and_clustered_spectrum(hip_ctx):
     a = (q * base) @ q.conj().T
     a = (a + a.conj().T) / 2
     w, v, fused = hip_ctx.hermitian_eigen(a, 320)
-    assert fused
+    assert fused == 1
     check_pairs(a, w, v, 320, np.sort(base))
 
 
+def _from_spectrum(lam, seed):
+    n = len(lam)
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    a = (q * lam) @ q.conj().T
+    return (a + a.conj().T) / 2
+
+
+@pytest.mark.parametrize("case", ["12-fold", "split-1e-9", "split-1e-6", "6-fold-960"])
+def test_inverse_iteration_clusters(hip_ctx, case):
+    """Spectra whose lowest eigenvalues come in groups: exact 12-fold and
+    6-fold multiplicities (one inverse-iteration cluster each, Gram-Schmidt
+    inside it) and triplets split by 1e-9 or 1e-6 (inside or outside the
+    clustering gap of 1e-7 |T|). The inverse-iteration path must accept them
+    (no divide-and-conquer fallback) with LAPACK-level residuals and an
+    orthonormal basis."""
+    rng = np.random.default_rng(11)
+    if case == "12-fold":
+        lam = np.concatenate([np.repeat(rng.uniform(-1, 1, 20), 12), rng.uniform(1, 15, 400)])
+    elif case == "6-fold-960":
+        lam = np.concatenate([np.repeat(rng.uniform(-1, 1, 40), 6), rng.uniform(1, 15, 720)])
+    else:
+        split = 1e-9 if case == "split-1e-9" else 1e-6
+        lam = np.concatenate([np.repeat(rng.uniform(-1, 1, 60), 3) + split * rng.standard_normal(180),
+                              rng.uniform(1, 15, 300)])
+    lam = np.sort(lam)
+    a = _from_spectrum(lam, 5)
+    k = 320 if len(lam) >= 640 else 240
+    w, v, fused = hip_ctx.hermitian_eigen(a, k)
+    assert fused == 1
+    check_pairs(a, w, v, k, np.linalg.eigvalsh(a))
+
+
+def test_glued_wilkinson_tridiagonal(hip_ctx):
+    """Twenty copies of Wilkinson's W21 glued by 1e-14 (pairs of eigenvalues
+    equal to many digits, twenty-fold near-multiplicities across copies): the
+    classic hard case for inverse iteration."""
+    w21 = np.abs(np.arange(-10, 11)).astype(float)
+    d = np.tile(w21, 20)
+    off = np.ones(d.size - 1)
+    off[20::21] = 1e-14
+    t = (np.diag(d) + np.diag(off, 1) + np.diag(off, -1)).astype(complex)
+    w, v, fused = hip_ctx.hermitian_eigen(t, 200)
+    assert fused == 1
+    check_pairs(t, w, v, 200, np.linalg.eigvalsh(t))
+
+
 def test_structured_inputs(hip_ctx):
-    """Diagonal input (every reflector is the identity, tau = 0) and a
-    tridiagonal input with complex off-diagonal (order-1 complex reflectors)."""
+    """Diagonal input (every column already reduced), a tridiagonal input with
+    complex off-diagonal (order-1 complex reflectors) and a real tridiagonal
+    input of order 300 (every column already reduced; the reduction's
+    reflectors are then sign flips, which the back-transformation must keep
+    finite)."""
     d = np.linspace(2, -1, 50)
     w, v, _ = hip_ctx.hermitian_eigen(np.diag(d).astype(complex), 50)
     check_pairs(np.diag(d).astype(complex), w, v, 50, np.sort(d))
@@ -58,6 +109,12 @@ def test_structured_inputs(hip_ctx):
     t = np.diag(rng.standard_normal(80)).astype(complex) + np.diag(off, -1) + np.diag(off.conj(), 1)
     w, v, _ = hip_ctx.hermitian_eigen(t, 80)
     check_pairs(t, w, v, 80, np.linalg.eigvalsh(t))
+    d = rng.standard_normal(300)
+    off = rng.standard_normal(299)
+    t = (np.diag(d) + np.diag(off, 1) + np.diag(off, -1)).astype(complex)
+    w, v, path = hip_ctx.hermitian_eigen(t, 100)
+    assert path == 1
+    check_pairs(t, w, v, 100, np.linalg.eigvalsh(t))
 
 
 def test_repeated_calls_reuse_state(hip_ctx):
