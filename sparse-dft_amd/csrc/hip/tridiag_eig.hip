@@ -19,6 +19,7 @@
 
 #include "tridiag_core.hpp"
 #include "tridiag_eig.hpp"
+#include "wave_sum.hpp"
 
 namespace sdft_hip {
 namespace {
@@ -28,10 +29,7 @@ constexpr int kValueWaves = 4;
 constexpr int kRounds = 24;
 constexpr int kSolves = 3;
 
-__device__ inline double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
-}
+__device__ inline double wave_sum(double v) { return wave::sum(v); }
 __device__ inline double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
   return v;
@@ -187,7 +185,98 @@ __global__ __launch_bounds__(kWave) void k_tridiag_vectors(int n, const double* 
   if (bad && lane == 0) *status = 1;
 }
 
+// Z <- Q Z with Q = H_0 H_1 ... H_{n-2} of the reduction (LAPACK zhetrd, uplo
+// 'L'): H_j = I - tau_j v_j v_j^H, v_j zero above row j + 1, v_j(j + 1) = 1,
+// rows j + 2 .. n - 1 in column j of refl. One wavefront per column of Z keeps
+// the column in registers (row s * 64 + lane in slot s) and applies H_{n-2},
+// ..., H_0 in turn, loading the next reflector while it applies the current
+// one; slots entirely above row j + 1 are skipped. Columns are independent:
+// no barrier and no exchange between wavefronts.
+template <int NS>
+__global__ __launch_bounds__(256) void k_back_transform(int n, int k, const double* __restrict__ refl, long ldr,
+                                                        const double* __restrict__ tau, double* __restrict__ z,
+                                                        long ldz) {
+  const int lane = threadIdx.x % kWave;
+  const int col = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (col >= k) return;
+  double* zc = z + 2 * static_cast<long>(col) * ldz;
+  double zr[NS], zi[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int i = s * kWave + lane;
+    zr[s] = i < n ? zc[2 * i] : 0.0;
+    zi[s] = i < n ? zc[2 * i + 1] : 0.0;
+  }
+  double vr[NS], vi[NS], nr[NS], ni[NS];
+  auto load = [&](int j, double* ur, double* ui) {
+    const double* vc = refl + 2 * static_cast<long>(j) * ldr;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      ur[s] = 0.0;
+      ui[s] = 0.0;
+      if ((s + 1) * kWave > j + 1) {
+        const int i = s * kWave + lane;
+        if (i >= j + 2 && i < n) {
+          ur[s] = vc[2 * i];
+          ui[s] = vc[2 * i + 1];
+        } else if (i == j + 1) {
+          ur[s] = 1.0;
+        }
+      }
+    }
+  };
+  if (n >= 2) load(n - 2, vr, vi);
+  for (int j = n - 2; j >= 0; --j) {
+    if (j > 0) load(j - 1, nr, ni);
+    const double tr = tau[2 * j], ti = tau[2 * j + 1];
+    double dr = 0.0, di = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      if ((s + 1) * kWave > j + 1) {
+        dr += vr[s] * zr[s] + vi[s] * zi[s];
+        di += vr[s] * zi[s] - vi[s] * zr[s];
+      }
+    }
+    dr = wave::sum(dr);
+    di = wave::sum(di);
+    const double hr = tr * dr - ti * di, hi = tr * di + ti * dr;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      if ((s + 1) * kWave > j + 1) {
+        zr[s] -= vr[s] * hr - vi[s] * hi;
+        zi[s] -= vr[s] * hi + vi[s] * hr;
+      }
+      vr[s] = nr[s];
+      vi[s] = ni[s];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int i = s * kWave + lane;
+    if (i < n) {
+      zc[2 * i] = zr[s];
+      zc[2 * i + 1] = zi[s];
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t launch_back_transform(int n, int k, const double* refl, long ldr, const double* tau, double* z, long ldz,
+                                 hipStream_t s) {
+  if (n < 1 || n > kBackTransformMaxOrder || k < 0 || ldr < n || ldz < n) return hipErrorInvalidValue;
+  if (k == 0 || n < 2) return hipSuccess;
+  const dim3 grid((k + 3) / 4), block(4 * kWave);
+  if (n <= 128)
+    hipLaunchKernelGGL(k_back_transform<2>, grid, block, 0, s, n, k, refl, ldr, tau, z, ldz);
+  else if (n <= 256)
+    hipLaunchKernelGGL(k_back_transform<4>, grid, block, 0, s, n, k, refl, ldr, tau, z, ldz);
+  else if (n <= 512)
+    hipLaunchKernelGGL(k_back_transform<8>, grid, block, 0, s, n, k, refl, ldr, tau, z, ldz);
+  else
+    hipLaunchKernelGGL(k_back_transform<16>, grid, block, 0, s, n, k, refl, ldr, tau, z, ldz);
+  return hipGetLastError();
+}
 
 hipError_t launch_tridiag_values(int n, const double* d, const double* e, double* w, int* status, hipStream_t s) {
   if (n < 1 || n > kTridiagEigMaxOrder) return hipErrorInvalidValue;

@@ -26,4 +26,13 @@ hipError_t launch_tridiag_values(int n, const double* d, const double* e, double
 hipError_t launch_tridiag_vectors(int n, const double* d, const double* e, const double* w, int k, double* z,
                                   long ldz, int* status, hipStream_t s);
 
+// Largest order of the register-resident back-transformation (16 rows per lane).
+constexpr int kBackTransformMaxOrder = 1024;
+
+// z (n x k complex, ld ldz) <- Q z for the Q of the tridiagonalisation
+// (reflector tails refl, ld ldr, and tau as launch_hetrd leaves them): the
+// product rocSOLVER zunmtr forms with side 'L', uplo 'L', no transpose.
+hipError_t launch_back_transform(int n, int k, const double* refl, long ldr, const double* tau, double* z, long ldz,
+                                 hipStream_t s);
+
 }  // namespace sdft_hip

@@ -108,12 +108,14 @@ bool HermitianEigensolver::inverse_iteration(long m, long k, double* vecs, std::
   check(launch_tridiag_values(int(m), d, e, w, status, s), "tridiagonal eigenvalues");
   check(hipMemsetAsync(defect_bits, 0, sizeof(unsigned long long), s), "memset");
   if (k > 0) {
-    auto* Zm = reinterpret_cast<rocblas_double_complex*>(z_.p);
     check(launch_tridiag_vectors(int(m), d, e, w, int(k), z_.p, m, status, s), "tridiagonal eigenvectors");
-    if (m > 1)
+    if (m <= kBackTransformMaxOrder)
+      check(launch_back_transform(int(m), int(k), refl_.p, m, tau, z_.p, m, s), "back-transformation");
+    else
       check(rocsolver_zunmtr(ctx_.blas(), rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
                              rocblas_int(m), rocblas_int(k), reinterpret_cast<rocblas_double_complex*>(refl_.p),
-                             rocblas_int(m), reinterpret_cast<rocblas_double_complex*>(tau), Zm, rocblas_int(m)),
+                             rocblas_int(m), reinterpret_cast<rocblas_double_complex*>(tau),
+                             reinterpret_cast<rocblas_double_complex*>(z_.p), rocblas_int(m)),
             "rocsolver_zunmtr");
     const std::size_t kk = std::size_t(2 * k * k);
     orth_.reserve(4 * kk);

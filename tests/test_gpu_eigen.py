@@ -24,7 +24,7 @@ def check_pairs(a, w, v, k, ref):
         assert np.abs(v.conj().T @ v - np.eye(k)).max() <= 1e-11
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 17, 64, 160, 333, 640, 960])
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 64, 129, 160, 333, 640, 960, 1100])
 def test_random_hermitian(hip_ctx, n):
     a = random_hermitian(n, n)
     k = min(n, 320)
