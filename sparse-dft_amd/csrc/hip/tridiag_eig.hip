@@ -15,6 +15,7 @@
 //    Members of a cluster get shifts at least 10 eps |x| apart (LAPACK dstein).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 
 #include "tridiag_core.hpp"
@@ -63,7 +64,8 @@ __device__ inline Bounds gershgorin(const double* d, const double* e, int n) {
 
 __global__ __launch_bounds__(kValueWaves * kWave) void k_tridiag_values(int n, const double* __restrict__ d,
                                                                         const double* __restrict__ e,
-                                                                        double* __restrict__ w, int* status) {
+                                                                        int count, double* __restrict__ w,
+                                                                        int* status) {
   extern __shared__ double lds[];
   double* sd = lds;
   double* se = sd + n;
@@ -78,7 +80,7 @@ __global__ __launch_bounds__(kValueWaves * kWave) void k_tridiag_values(int n, c
   __syncthreads();
   const int lane = threadIdx.x % kWave;
   const int index = blockIdx.x * kValueWaves + threadIdx.x / kWave;
-  if (index >= n) return;
+  if (index >= count) return;
   const Bounds b = gershgorin(sd, se, n);
   double lo = b.gl, hi = b.gu;
   const double atol = DBL_EPSILON * b.tnorm;
@@ -188,68 +190,95 @@ __global__ __launch_bounds__(kWave) void k_tridiag_vectors(int n, const double* 
 // Z <- Q Z with Q = H_0 H_1 ... H_{n-2} of the reduction (LAPACK zhetrd, uplo
 // 'L'): H_j = I - tau_j v_j v_j^H, v_j zero above row j + 1, v_j(j + 1) = 1,
 // rows j + 2 .. n - 1 in column j of refl. One wavefront per column of Z keeps
-// the column in registers (row s * 64 + lane in slot s) and applies H_{n-2},
-// ..., H_0 in turn, loading the next reflector while it applies the current
-// one; slots entirely above row j + 1 are skipped. Columns are independent:
-// no barrier and no exchange between wavefronts.
+// the column in registers (row s * 64 + lane in slot s, NS slots) and applies
+// H_{n-2}, ..., H_0 in turn. The workgroup stages `chunk` reflectors at a time
+// in LDS, every load of a round issued before the first store, so that one
+// memory latency serves `chunk` steps. A step reads all NS slots of the
+// reflector from LDS at once and masks the rows above j + 1 with selects
+// (branch-free, so the reads are not serialised); columns are independent.
+constexpr int kBackWaves = 4;
+constexpr int kBackThreads = kBackWaves * kWave;
+constexpr int kBackMaxChunk = 8;
+constexpr int kBackRows = (kBackTransformMaxOrder + kBackThreads - 1) / kBackThreads;
+
 template <int NS>
-__global__ __launch_bounds__(256) void k_back_transform(int n, int k, const double* __restrict__ refl, long ldr,
-                                                        const double* __restrict__ tau, double* __restrict__ z,
-                                                        long ldz) {
-  const int lane = threadIdx.x % kWave;
-  const int col = blockIdx.x * 4 + threadIdx.x / kWave;
-  if (col >= k) return;
-  double* zc = z + 2 * static_cast<long>(col) * ldz;
+__global__ __launch_bounds__(kBackThreads) void k_back_transform(int n, int k, int chunk,
+                                                                 const double* __restrict__ refl, long ldr,
+                                                                 const double* __restrict__ tau,
+                                                                 double* __restrict__ z, long ldz) {
+  extern __shared__ __attribute__((aligned(16))) double lds[];
+  double2* stage = reinterpret_cast<double2*>(lds);
+  const int tid = threadIdx.x;
+  const int lane = tid % kWave;
+  const int col = blockIdx.x * kBackWaves + tid / kWave;
+  const bool active = col < k;  // inactive wavefronts still stage reflectors
+  double* zc = z + 2 * static_cast<long>(active ? col : 0) * ldz;
   double zr[NS], zi[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const int i = s * kWave + lane;
-    zr[s] = i < n ? zc[2 * i] : 0.0;
-    zi[s] = i < n ? zc[2 * i + 1] : 0.0;
+    zr[s] = active && i < n ? zc[2 * i] : 0.0;
+    zi[s] = active && i < n ? zc[2 * i + 1] : 0.0;
   }
-  double vr[NS], vi[NS], nr[NS], ni[NS];
-  auto load = [&](int j, double* ur, double* ui) {
-    const double* vc = refl + 2 * static_cast<long>(j) * ldr;
+  for (int j0 = n - 2; j0 >= 0; j0 -= chunk) {
+    const int count = j0 + 1 < chunk ? j0 + 1 : chunk;  // reflectors j0, j0 - 1, ..., j0 - count + 1
+    double2 buf[kBackMaxChunk][kBackRows];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      ur[s] = 0.0;
-      ui[s] = 0.0;
-      if ((s + 1) * kWave > j + 1) {
-        const int i = s * kWave + lane;
-        if (i >= j + 2 && i < n) {
-          ur[s] = vc[2 * i];
-          ui[s] = vc[2 * i + 1];
-        } else if (i == j + 1) {
-          ur[s] = 1.0;
+    for (int c = 0; c < kBackMaxChunk; ++c) {
+      const int j = j0 - c;
+      const double2* vc = reinterpret_cast<const double2*>(refl + 2 * static_cast<long>(j) * ldr);
+#pragma unroll
+      for (int r = 0; r < kBackRows; ++r) {
+        const int i = j + 2 + tid + r * kBackThreads;
+        buf[c][r] = c < count && i < n ? vc[i] : make_double2(0.0, 0.0);
+      }
+    }
+    __syncthreads();  // the previous round's reflectors are consumed
+#pragma unroll
+    for (int c = 0; c < kBackMaxChunk; ++c) {
+      const int j = j0 - c;
+      if (c < count) {
+        if (tid == 0) stage[static_cast<long>(c) * n + j + 1] = make_double2(1.0, 0.0);
+#pragma unroll
+        for (int r = 0; r < kBackRows; ++r) {
+          const int i = j + 2 + tid + r * kBackThreads;
+          if (i < n) stage[static_cast<long>(c) * n + i] = buf[c][r];
         }
       }
     }
-  };
-  if (n >= 2) load(n - 2, vr, vi);
-  for (int j = n - 2; j >= 0; --j) {
-    if (j > 0) load(j - 1, nr, ni);
-    const double tr = tau[2 * j], ti = tau[2 * j + 1];
-    double dr = 0.0, di = 0.0;
+    __syncthreads();
+    if (!active) continue;
+    for (int c = 0; c < count; ++c) {
+      const int j = j0 - c;
+      const double2* v = stage + static_cast<long>(c) * n;
+      const double tr = tau[2 * j], ti = tau[2 * j + 1];
+      double2 vv[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      if ((s + 1) * kWave > j + 1) {
-        dr += vr[s] * zr[s] + vi[s] * zi[s];
-        di += vr[s] * zi[s] - vi[s] * zr[s];
+      for (int s = 0; s < NS; ++s) {
+        const int i = s * kWave + lane;
+        vv[s] = v[i < n ? i : n - 1];
       }
-    }
-    dr = wave::sum(dr);
-    di = wave::sum(di);
-    const double hr = tr * dr - ti * di, hi = tr * di + ti * dr;
+      double dr[4] = {0.0, 0.0, 0.0, 0.0}, di[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      if ((s + 1) * kWave > j + 1) {
-        zr[s] -= vr[s] * hr - vi[s] * hi;
-        zi[s] -= vr[s] * hi + vi[s] * hr;
+      for (int s = 0; s < NS; ++s) {
+        const int i = s * kWave + lane;
+        const bool live = i > j && i < n;
+        vv[s].x = live ? vv[s].x : 0.0;
+        vv[s].y = live ? vv[s].y : 0.0;
+        dr[s % 4] += vv[s].x * zr[s] + vv[s].y * zi[s];
+        di[s % 4] += vv[s].x * zi[s] - vv[s].y * zr[s];
       }
-      vr[s] = nr[s];
-      vi[s] = ni[s];
+      const double hr0 = wave::sum((dr[0] + dr[1]) + (dr[2] + dr[3]));
+      const double hi0 = wave::sum((di[0] + di[1]) + (di[2] + di[3]));
+      const double hr = tr * hr0 - ti * hi0, hi = tr * hi0 + ti * hr0;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        zr[s] -= vv[s].x * hr - vv[s].y * hi;
+        zi[s] -= vv[s].x * hi + vv[s].y * hr;
+      }
     }
   }
+  if (!active) return;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const int i = s * kWave + lane;
@@ -266,23 +295,34 @@ hipError_t launch_back_transform(int n, int k, const double* refl, long ldr, con
                                  hipStream_t s) {
   if (n < 1 || n > kBackTransformMaxOrder || k < 0 || ldr < n || ldz < n) return hipErrorInvalidValue;
   if (k == 0 || n < 2) return hipSuccess;
-  const dim3 grid((k + 3) / 4), block(4 * kWave);
-  if (n <= 128)
-    hipLaunchKernelGGL(k_back_transform<2>, grid, block, 0, s, n, k, refl, ldr, tau, z, ldz);
-  else if (n <= 256)
-    hipLaunchKernelGGL(k_back_transform<4>, grid, block, 0, s, n, k, refl, ldr, tau, z, ldz);
-  else if (n <= 512)
-    hipLaunchKernelGGL(k_back_transform<8>, grid, block, 0, s, n, k, refl, ldr, tau, z, ldz);
-  else
-    hipLaunchKernelGGL(k_back_transform<16>, grid, block, 0, s, n, k, refl, ldr, tau, z, ldz);
+  // reflectors staged per round: as many as 144 KiB of LDS hold, at most 8
+  const int chunk = std::max(1, std::min(kBackMaxChunk, int((144 * 1024) / (16 * size_t(n)))));
+  const size_t lds = 16 * size_t(chunk) * size_t(n);
+  const dim3 grid((k + kBackWaves - 1) / kBackWaves), block(kBackThreads);
+  const int slots = (n + kWave - 1) / kWave;
+#define SDFT_BACK(NS)                                                                                      \
+  if (slots <= NS) {                                                                                       \
+    hipLaunchKernelGGL(k_back_transform<NS>, grid, block, lds, s, n, k, chunk, refl, ldr, tau, z, ldz);   \
+    return hipGetLastError();                                                                              \
+  }
+  SDFT_BACK(2)
+  SDFT_BACK(4)
+  SDFT_BACK(6)
+  SDFT_BACK(8)
+  SDFT_BACK(10)
+  SDFT_BACK(12)
+  SDFT_BACK(14)
+  SDFT_BACK(16)
+#undef SDFT_BACK
   return hipGetLastError();
 }
 
-hipError_t launch_tridiag_values(int n, const double* d, const double* e, double* w, int* status, hipStream_t s) {
-  if (n < 1 || n > kTridiagEigMaxOrder) return hipErrorInvalidValue;
+hipError_t launch_tridiag_values(int n, const double* d, const double* e, int count, double* w, int* status,
+                                 hipStream_t s) {
+  if (n < 1 || n > kTridiagEigMaxOrder || count < 1 || count > n) return hipErrorInvalidValue;
   const size_t lds = 3 * sizeof(double) * size_t(n);
-  const int blocks = (n + kValueWaves - 1) / kValueWaves;
-  hipLaunchKernelGGL(k_tridiag_values, dim3(blocks), dim3(kValueWaves * kWave), lds, s, n, d, e, w, status);
+  const int blocks = (count + kValueWaves - 1) / kValueWaves;
+  hipLaunchKernelGGL(k_tridiag_values, dim3(blocks), dim3(kValueWaves * kWave), lds, s, n, d, e, count, w, status);
   return hipGetLastError();
 }
 

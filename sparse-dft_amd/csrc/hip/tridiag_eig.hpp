@@ -13,10 +13,11 @@ constexpr int kTridiagEigMaxOrder = 2048;
 // during the iteration.
 constexpr double kTridiagClusterGap = 1e-7;
 
-// d (n), e (n - 1): T. w (n): all eigenvalues, ascending to within the
-// bisection tolerance (adjacent values of a degenerate group may differ by an
-// ulp in either order). status: one int, set to 0.
-hipError_t launch_tridiag_values(int n, const double* d, const double* e, double* w, int* status, hipStream_t s);
+// d (n), e (n - 1): T. w (count): the lowest count eigenvalues, ascending to
+// within the bisection tolerance (adjacent values of a degenerate group may
+// differ by an ulp in either order). status: one int, set to 0.
+hipError_t launch_tridiag_values(int n, const double* d, const double* e, int count, double* w, int* status,
+                                 hipStream_t s);
 
 // Eigenvectors of the lowest k eigenvalues w (from launch_tridiag_values on
 // the same stream) into the real parts of the complex columns of z (ld ldz,

@@ -701,7 +701,7 @@ int sdft_hip_tridiagonal_eigen(sdft_hip_context* ctx, size_t n, const double* d,
     z.reserve(2 * n * std::max<size_t>(k, 1));
     sdft_hip::check(hipMemcpyAsync(dd, d, 8 * n, hipMemcpyHostToDevice, s), "upload");
     if (n > 1) sdft_hip::check(hipMemcpyAsync(de, e, 8 * (n - 1), hipMemcpyHostToDevice, s), "upload");
-    sdft_hip::check(sdft_hip::launch_tridiag_values(int(n), dd, de, dw, st, s), "tridiagonal eigenvalues");
+    sdft_hip::check(sdft_hip::launch_tridiag_values(int(n), dd, de, int(n), dw, st, s), "tridiagonal eigenvalues");
     sdft_hip::check(sdft_hip::launch_tridiag_vectors(int(n), dd, de, dw, int(k), z.p, long(n), st, s),
                     "tridiagonal eigenvectors");
     std::vector<double> zh(2 * n * k);

@@ -170,7 +170,7 @@ class Solver {
     ctx_.zgemm(f64, 0, 0, m, m, m, 1.0, t, m, mm, m, e2, m, nullptr, 0, 0.0);
     check(launch_hermitise(e2, m, m, e, m, s_), "hermitise");
     const long k = std::min(vectors, m);
-    std::vector<double> w = eig_->solve(e, m, k, t);
+    std::vector<double> w = eig_->solve(e, m, k, t, k);
     ctx_.zgemm(f64, 0, 0, m, k, m, 1.0, mm, m, t, m, a, m, nullptr, 0, 0.0);
     if (p_.timing) { sync(); stats_.seconds_ritz += now() - t0; }
     return w;

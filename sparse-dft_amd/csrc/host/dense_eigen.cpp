@@ -44,8 +44,9 @@ HermitianEigensolver::HermitianEigensolver(Context& ctx, long max_order) : ctx_(
   work_.reserve(hetrd_workspace_bytes(int(std::min<long>(max_order, kHetrdMaxOrder))));
 }
 
-std::vector<double> HermitianEigensolver::solve(const double* a, long m, long k, double* vecs) {
+std::vector<double> HermitianEigensolver::solve(const double* a, long m, long k, double* vecs, long values) {
   if (m < 1 || m > max_order_ || k < 0 || k > m) throw std::invalid_argument("HIP: invalid Hermitian eigensolve shape");
+  const long count = values < 0 ? m : std::max(std::max(k, std::min(values, m)), 1L);
   hipStream_t s = ctx_.stream();
   auto* Zm = reinterpret_cast<rocblas_double_complex*>(z_.p);
   int* info = reinterpret_cast<int*>(small_.p);
@@ -59,7 +60,7 @@ std::vector<double> HermitianEigensolver::solve(const double* a, long m, long k,
     ++fused_;
     if (!use_stedc() && m <= kTridiagEigMaxOrder) {
       std::vector<double> w;
-      if (inverse_iteration(m, k, vecs, w)) return w;
+      if (inverse_iteration(m, k, count, vecs, w)) return w;
       ++fallbacks_;
     }
     check(rocsolver_zstedc(ctx_.blas(), rocblas_evect_tridiagonal, rocblas_int(m), d, e, Zm, rocblas_int(m), info),
@@ -88,16 +89,19 @@ std::vector<double> HermitianEigensolver::solve(const double* a, long m, long k,
   int status = 0;
   std::memcpy(&status, h.data(), sizeof(int));
   if (status) throw std::runtime_error("HIP: Hermitian eigensolve failed");
-  return std::vector<double>(h.begin() + 8, h.end());
+  return std::vector<double>(h.begin() + 8, h.begin() + 8 + count);
 }
 
-// Tridiagonal eigenpairs by multisection and inverse iteration, the wanted
-// vectors back-transformed (zunmtr) and symmetrically orthonormalised:
+// Tridiagonal eigenpairs by multisection and inverse iteration (the lowest
+// `count` values, the lowest k vectors), the vectors back-transformed
+// (register-resident kernel, zunmtr above its order) and symmetrically
+// orthonormalised:
 // vecs = Z (I + E)^(-1/2) to second order, E = Z^H Z - I. Mixing two
 // eigenvectors by their overlap c, which inverse iteration bounds by
 // O(eps |T| / gap), changes their residuals by c gap = O(eps |T|). Returns false
 // (outputs invalid) when an iteration did not stay finite or |E| > kMaxDefect.
-bool HermitianEigensolver::inverse_iteration(long m, long k, double* vecs, std::vector<double>& values) {
+bool HermitianEigensolver::inverse_iteration(long m, long k, long count, double* vecs,
+                                             std::vector<double>& values) {
   hipStream_t s = ctx_.stream();
   int* status = reinterpret_cast<int*>(small_.p);
   auto* defect_bits = reinterpret_cast<unsigned long long*>(small_.p + 2);
@@ -105,7 +109,7 @@ bool HermitianEigensolver::inverse_iteration(long m, long k, double* vecs, std::
   double* e = d + m;
   double* tau = e + m;
   double* w = tau + 2 * m;
-  check(launch_tridiag_values(int(m), d, e, w, status, s), "tridiagonal eigenvalues");
+  check(launch_tridiag_values(int(m), d, e, int(count), w, status, s), "tridiagonal eigenvalues");
   check(hipMemsetAsync(defect_bits, 0, sizeof(unsigned long long), s), "memset");
   if (k > 0) {
     check(launch_tridiag_vectors(int(m), d, e, w, int(k), z_.p, m, status, s), "tridiagonal eigenvectors");
@@ -130,8 +134,8 @@ bool HermitianEigensolver::inverse_iteration(long m, long k, double* vecs, std::
     check(launch_series_metric(ge, ge2, k, k, gm, k, s), "series metric");
     ctx_.zgemm(f64, 0, 0, m, k, k, 1.0, z_.p, m, gm, k, vecs, m, nullptr, 0, 0.0);
   }
-  values.assign(std::size_t(m), 0.0);
-  check(hipMemcpyAsync(values.data(), w, 8 * std::size_t(m), hipMemcpyDeviceToHost, s), "eigenvalues");
+  values.assign(std::size_t(count), 0.0);
+  check(hipMemcpyAsync(values.data(), w, 8 * std::size_t(count), hipMemcpyDeviceToHost, s), "eigenvalues");
   double head[3] = {0, 0, 0};
   check(hipMemcpyAsync(head, small_.p, sizeof(head), hipMemcpyDeviceToHost, s), "status");
   int aborted = 0;
@@ -147,7 +151,7 @@ bool HermitianEigensolver::inverse_iteration(long m, long k, double* vecs, std::
   double defect = 0.0;
   std::memcpy(&defect, &bits, sizeof(defect));
   bool finite = true;
-  for (long i = 0; i < m; ++i) finite = finite && std::isfinite(values[std::size_t(i)]);
+  for (long i = 0; i < count; ++i) finite = finite && std::isfinite(values[std::size_t(i)]);
   if (st || !(defect <= kMaxDefect) || !finite) {
     if (std::getenv("SDFT_HIP_TRIDIAG_VERBOSE"))
       std::fprintf(stderr, "tridiagonal inverse iteration rejected: m=%ld k=%ld status=%d defect=%.3e finite=%d\n", m,
@@ -155,7 +159,8 @@ bool HermitianEigensolver::inverse_iteration(long m, long k, double* vecs, std::
     return false;
   }
   // a degenerate group may come back an ulp out of order
-  for (long i = 1; i < m; ++i) values[std::size_t(i)] = std::max(values[std::size_t(i)], values[std::size_t(i - 1)]);
+  for (long i = 1; i < count; ++i)
+    values[std::size_t(i)] = std::max(values[std::size_t(i)], values[std::size_t(i - 1)]);
   return true;
 }
 

@@ -14,9 +14,9 @@ class HermitianEigensolver {
  public:
   HermitianEigensolver(Context& ctx, long max_order);
   // a: m x m device matrix (ld m), exactly Hermitian, left unchanged. Returns
-  // all m eigenvalues ascending; the eigenvectors of the lowest k go to vecs
-  // (device, ld m).
-  std::vector<double> solve(const double* a, long m, long k, double* vecs);
+  // the lowest max(k, values) eigenvalues ascending (all m when values < 0);
+  // the eigenvectors of the lowest k go to vecs (device, ld m).
+  std::vector<double> solve(const double* a, long m, long k, double* vecs, long values = -1);
   long launches_fused() const { return fused_; }
   // inverse-iteration solves that failed their checks and were redone by
   // divide and conquer; tridiagonal solves by divide and conquer in total
@@ -26,7 +26,7 @@ class HermitianEigensolver {
  private:
   Context& ctx_;
   long max_order_;
-  bool inverse_iteration(long m, long k, double* vecs, std::vector<double>& values);
+  bool inverse_iteration(long m, long k, long count, double* vecs, std::vector<double>& values);
   DevBuf<double> refl_, z_, small_, orth_;
   DevBuf<char> work_;
   long fused_ = 0, fallbacks_ = 0, stedc_ = 0;
