@@ -192,14 +192,16 @@ __global__ __launch_bounds__(kWave) void k_tridiag_vectors(int n, const double* 
 // rows j + 2 .. n - 1 in column j of refl. One wavefront per column of Z keeps
 // the column in registers (row s * 64 + lane in slot s, NS slots) and applies
 // H_{n-2}, ..., H_0 in turn. The workgroup stages `chunk` reflectors at a time
-// in LDS, every load of a round issued before the first store, so that one
-// memory latency serves `chunk` steps. A step reads all NS slots of the
-// reflector from LDS at once and masks the rows above j + 1 with selects
-// (branch-free, so the reads are not serialised); columns are independent.
+// in LDS as full NS * 64-row vectors (zeros above row j + 1 and below row
+// n - 1); the loads of the next round are issued before the steps of the
+// current one, so their latency is hidden behind `chunk` steps. A step reads
+// all NS slots of the reflector from LDS at once, branch- and mask-free;
+// columns are independent.
 constexpr int kBackWaves = 4;
 constexpr int kBackThreads = kBackWaves * kWave;
-constexpr int kBackMaxChunk = 8;
-constexpr int kBackRows = (kBackTransformMaxOrder + kBackThreads - 1) / kBackThreads;
+// reflectors per round: the register buffer of the next round (rows per
+// thread x chunk) stays live during the current round's steps
+__host__ __device__ constexpr int back_chunk(int ns) { return ns <= 8 ? 8 : (ns <= 12 ? 6 : 4); }
 
 template <int NS>
 __global__ __launch_bounds__(kBackThreads) void k_back_transform(int n, int k, int chunk,
@@ -208,6 +210,9 @@ __global__ __launch_bounds__(kBackThreads) void k_back_transform(int n, int k, i
                                                                  double* __restrict__ z, long ldz) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
   double2* stage = reinterpret_cast<double2*>(lds);
+  constexpr int rows = NS * kWave;  // staged length of one reflector
+  constexpr int per_thread = (rows + kBackThreads - 1) / kBackThreads;
+  constexpr int max_chunk = back_chunk(NS);
   const int tid = threadIdx.x;
   const int lane = tid % kWave;
   const int col = blockIdx.x * kBackWaves + tid / kWave;
@@ -220,51 +225,50 @@ __global__ __launch_bounds__(kBackThreads) void k_back_transform(int n, int k, i
     zr[s] = active && i < n ? zc[2 * i] : 0.0;
     zi[s] = active && i < n ? zc[2 * i + 1] : 0.0;
   }
-  for (int j0 = n - 2; j0 >= 0; j0 -= chunk) {
-    const int count = j0 + 1 < chunk ? j0 + 1 : chunk;  // reflectors j0, j0 - 1, ..., j0 - count + 1
-    double2 buf[kBackMaxChunk][kBackRows];
+  // the round starting at reflector j0 holds j0, j0 - 1, ..., j0 - count + 1
+  double2 buf[max_chunk][per_thread];
+  auto fetch = [&](int j0) {
+    const int count = j0 + 1 < chunk ? j0 + 1 : chunk;
 #pragma unroll
-    for (int c = 0; c < kBackMaxChunk; ++c) {
+    for (int c = 0; c < max_chunk; ++c) {
       const int j = j0 - c;
       const double2* vc = reinterpret_cast<const double2*>(refl + 2 * static_cast<long>(j) * ldr);
 #pragma unroll
-      for (int r = 0; r < kBackRows; ++r) {
-        const int i = j + 2 + tid + r * kBackThreads;
-        buf[c][r] = c < count && i < n ? vc[i] : make_double2(0.0, 0.0);
+      for (int r = 0; r < per_thread; ++r) {
+        const int i = tid + r * kBackThreads;
+        double2 v = make_double2(i == j + 1 ? 1.0 : 0.0, 0.0);
+        if (c < count && i > j + 1 && i < n) v = vc[i];
+        buf[c][r] = v;
       }
     }
+  };
+  fetch(n - 2);
+  for (int j0 = n - 2; j0 >= 0; j0 -= chunk) {
+    const int count = j0 + 1 < chunk ? j0 + 1 : chunk;
     __syncthreads();  // the previous round's reflectors are consumed
 #pragma unroll
-    for (int c = 0; c < kBackMaxChunk; ++c) {
-      const int j = j0 - c;
+    for (int c = 0; c < max_chunk; ++c) {
       if (c < count) {
-        if (tid == 0) stage[static_cast<long>(c) * n + j + 1] = make_double2(1.0, 0.0);
 #pragma unroll
-        for (int r = 0; r < kBackRows; ++r) {
-          const int i = j + 2 + tid + r * kBackThreads;
-          if (i < n) stage[static_cast<long>(c) * n + i] = buf[c][r];
+        for (int r = 0; r < per_thread; ++r) {
+          const int i = tid + r * kBackThreads;
+          if (i < rows) stage[static_cast<long>(c) * rows + i] = buf[c][r];
         }
       }
     }
     __syncthreads();
+    if (j0 >= chunk) fetch(j0 - chunk);  // in flight during this round's steps
     if (!active) continue;
     for (int c = 0; c < count; ++c) {
       const int j = j0 - c;
-      const double2* v = stage + static_cast<long>(c) * n;
+      const double2* v = stage + static_cast<long>(c) * rows;
       const double tr = tau[2 * j], ti = tau[2 * j + 1];
       double2 vv[NS];
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const int i = s * kWave + lane;
-        vv[s] = v[i < n ? i : n - 1];
-      }
+      for (int s = 0; s < NS; ++s) vv[s] = v[s * kWave + lane];
       double dr[4] = {0.0, 0.0, 0.0, 0.0}, di[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
-        const int i = s * kWave + lane;
-        const bool live = i > j && i < n;
-        vv[s].x = live ? vv[s].x : 0.0;
-        vv[s].y = live ? vv[s].y : 0.0;
         dr[s % 4] += vv[s].x * zr[s] + vv[s].y * zi[s];
         di[s % 4] += vv[s].x * zi[s] - vv[s].y * zr[s];
       }
@@ -295,11 +299,10 @@ hipError_t launch_back_transform(int n, int k, const double* refl, long ldr, con
                                  hipStream_t s) {
   if (n < 1 || n > kBackTransformMaxOrder || k < 0 || ldr < n || ldz < n) return hipErrorInvalidValue;
   if (k == 0 || n < 2) return hipSuccess;
-  // reflectors staged per round: as many as 144 KiB of LDS hold, at most 8
-  const int chunk = std::max(1, std::min(kBackMaxChunk, int((144 * 1024) / (16 * size_t(n)))));
-  const size_t lds = 16 * size_t(chunk) * size_t(n);
+  const int slots = (n + kWave - 1) / kWave + ((n + kWave - 1) / kWave) % 2;  // the instantiated NS
+  const int chunk = back_chunk(slots);
+  const size_t lds = 16 * size_t(chunk) * size_t(slots) * kWave;
   const dim3 grid((k + kBackWaves - 1) / kBackWaves), block(kBackThreads);
-  const int slots = (n + kWave - 1) / kWave;
 #define SDFT_BACK(NS)                                                                                      \
   if (slots <= NS) {                                                                                       \
     hipLaunchKernelGGL(k_back_transform<NS>, grid, block, lds, s, n, k, chunk, refl, ldr, tau, z, ldz);   \
