@@ -123,3 +123,34 @@ def test_repeated_calls_reuse_state(hip_ctx):
         a = random_hermitian(n, 100 + n)
         w, _, _ = hip_ctx.hermitian_eigen(a, 0)
         assert np.abs(w - np.linalg.eigvalsh(a)).max() <= 1e-12 * max(1, np.abs(w).max()) * 4
+
+
+def test_tridiagonal_stage_matches_host_core(hip_ctx):
+    """The device tridiagonal stage (multisection + inverse iteration) against
+    the host build of the same arithmetic (tests/native/tridiag_core_check.cpp):
+    eigenvalues to the bisection tolerance, vectors spanning the same
+    eigenspaces (equal up to sign outside clusters)."""
+    import ctypes as ct
+    import pathlib
+    lib = ct.CDLL(str(pathlib.Path(__file__).parent / "native" / "libtridiag_core_check.so"))
+    rng = np.random.default_rng(21)
+    w21 = np.abs(np.arange(-10, 11)).astype(float)
+    glued_d = np.tile(w21, 10)
+    glued_e = np.ones(glued_d.size - 1)
+    glued_e[20::21] = 1e-14
+    for d, e, k in ((rng.standard_normal(400), rng.standard_normal(399), 130), (glued_d, glued_e, 100)):
+        n = d.size
+        w, z, status = hip_ctx.tridiagonal_eigen(d, e, k)
+        assert status == 0
+        wh = np.zeros(n)
+        p = lambda a: a.ctypes.data_as(ct.c_void_p)  # noqa: E731
+        lib.tridiag_core_values(n, p(d), p(e), n, p(wh))
+        zh = np.zeros((k, n))
+        assert lib.tridiag_core_vectors(n, p(d), p(e), p(wh), k, ct.c_double(1e-7), p(zh)) > 0
+        zh = zh.T
+        t = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+        tn = np.abs(wh).max()
+        assert np.abs(w - wh).max() <= 2e-15 * tn
+        # same eigenspaces: projector difference of the k-dimensional spans
+        assert np.abs(z @ (z.T @ zh) - zh).max() <= 1e-10
+        assert np.abs(t @ z - z * w[:k]).max() <= 1e-14 * tn
