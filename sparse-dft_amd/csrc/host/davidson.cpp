@@ -4,6 +4,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cstring>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -71,7 +72,10 @@ class Solver {
   Precision search_prec() const { return exact_ ? Precision::FP64 : Precision::FP32; }
   Precision prec(bool exact) const { return exact ? Precision::FP64 : search_prec(); }
   double* col(DevBuf<double>& b, long j, long ld) { return b.p + 2 * j * ld; }
-  void sync() { ctx_.sync(); }
+  void sync() {
+    ctx_.sync();
+    stage_.reset();
+  }
   void gemm(bool exact, int aop, int bop, long M, long N, long K, double alpha, const double* A, long lda,
             const double* B, long ldb, double* Cm, long ldc, const double* D = nullptr, long ldd = 0,
             double beta = 0.0) {
@@ -91,16 +95,34 @@ class Solver {
   }
   std::vector<double> to_host(const double* d, long count) {
     std::vector<double> h(count);
-    check(hipMemcpyAsync(h.data(), d, 8 * std::size_t(count), hipMemcpyDeviceToHost, s_), "to host");
+    const std::size_t bytes = 8 * std::size_t(count);
+    void* slot = stage_.take(bytes);
+    if (!slot && bytes <= stage_.capacity()) {
+      sync();
+      slot = stage_.take(bytes);
+    }
+    check(hipMemcpyAsync(slot ? slot : h.data(), d, bytes, hipMemcpyDeviceToHost, s_), "to host");
     sync();
+    if (slot) std::memcpy(h.data(), slot, bytes);
     return h;
   }
-  // small host -> device transfers of control scalars; synchronous so that
-  // the host source may be reused immediately
-  void upload(double* d, const double* h, long count) {
-    check(hipMemcpyAsync(d, h, 8 * std::size_t(count), hipMemcpyHostToDevice, s_), "upload");
-    sync();
+  // small host -> device transfers of control scalars through pinned slots:
+  // the host source may be reused immediately and nothing waits
+  void upload_bytes(void* d, const void* h, std::size_t bytes) {
+    void* slot = stage_.take(bytes);
+    if (!slot && bytes <= stage_.capacity()) {
+      sync();
+      slot = stage_.take(bytes);
+    }
+    if (!slot) {
+      check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s_), "upload");
+      sync();
+      return;
+    }
+    std::memcpy(slot, h, bytes);
+    check(hipMemcpyAsync(d, slot, bytes, hipMemcpyHostToDevice, s_), "upload");
   }
+  void upload(double* d, const double* h, long count) { upload_bytes(d, h, 8 * std::size_t(count)); }
   std::vector<double> col_norms(const double* x, long ld, long cols) {
     double* o = small_.p;
     check(launch_col_norms(x, ld, n_, cols, o, s_), "norms");
@@ -249,6 +271,7 @@ class Solver {
   DevBuf<double> basis_, hb_, sb_, ch_, cs_, x_, sx_, hx_, res_;
   DevBuf<double> small_, w1_, w2_, w3_, w4_, m1_, m2_, m3_, rr_, rrb_, qbuf_, corr_, retained_;
   std::unique_ptr<HermitianEigensolver> eig_;
+  PinnedStaging& stage_ = ctx_.staging();
   long excl_ = 0;
   const double* excluded_ = nullptr;
   DevBuf<double> excluded_s_, excl_coef_;
@@ -740,7 +763,7 @@ DavidsonOutput Solver::run(const C* initial) {
         std::iota(src.begin(), src.end(), 0);
         check(launch_kinetic_weights(x_.p, n, n, R, op_.kinetic_device(), small_.p + 8, s_), "kinetic weights");
         int* srcd = reinterpret_cast<int*>(small_.p + 8 + R);
-        check(hipMemcpyAsync(srcd, src.data(), R * sizeof(int), hipMemcpyHostToDevice, s_), "src");
+        upload_bytes(srcd, src.data(), R * sizeof(int));
         check(launch_precondition(res_.p, n, srcd, R, small_.p + 8, op_.kinetic_device(), n, corr.p, n, s_),
               "precondition");
         ++it;
@@ -783,7 +806,7 @@ DavidsonOutput Solver::run(const C* initial) {
     if (!src.empty()) {
       check(launch_kinetic_weights(x_.p, n, n, R, op_.kinetic_device(), small_.p + 8, s_), "kinetic weights");
       int* srcd = reinterpret_cast<int*>(small_.p + 8 + R);
-      check(hipMemcpyAsync(srcd, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice, s_), "src");
+      upload_bytes(srcd, src.data(), src.size() * sizeof(int));
       check(launch_precondition(res_.p, n, srcd, long(src.size()), small_.p + 8, op_.kinetic_device(), n, corr.p, n, s_),
             "precondition");
       append_block(corr.p, long(src.size()));

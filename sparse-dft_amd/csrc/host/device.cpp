@@ -174,6 +174,27 @@ Context::Context(int device) : device_(device) {
   check(hipEventCreateWithFlags(&join_event_, hipEventDisableTiming), "hipEventCreate");
 }
 
+PinnedStaging::PinnedStaging(std::size_t bytes) : cap_(bytes) {
+  check(hipHostMalloc(reinterpret_cast<void**>(&base_), bytes, hipHostMallocDefault), "hipHostMalloc");
+}
+
+PinnedStaging::~PinnedStaging() {
+  if (base_) (void)hipHostFree(base_);
+}
+
+void* PinnedStaging::take(std::size_t bytes) {
+  const std::size_t aligned = (bytes + 255) / 256 * 256;
+  if (aligned > cap_ - used_) return nullptr;
+  void* p = base_ + used_;
+  used_ += aligned;
+  return p;
+}
+
+PinnedStaging& Context::staging() {
+  if (!staging_) staging_ = std::make_unique<PinnedStaging>(std::size_t(4) << 20);
+  return *staging_;
+}
+
 void Context::fork() {
   check(hipEventRecord(fork_event_, stream_), "fork record");
   check(hipStreamWaitEvent(aux_, fork_event_, 0), "fork wait");

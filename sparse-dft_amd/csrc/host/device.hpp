@@ -25,6 +25,26 @@ void check(rocblas_status s, const char* what);
 
 double gemm_algorithmic_bytes(long M, long N, long K, int a_bytes, bool reads_d);
 
+// Pinned host staging for the small control transfers of one stream: an
+// upload is copied into a fresh slot and queued without waiting, a download
+// lands in one; slots are handed out again only after reset(), which the
+// owner calls right after a full synchronisation of the stream.
+class PinnedStaging {
+ public:
+  explicit PinnedStaging(std::size_t bytes);
+  ~PinnedStaging();
+  PinnedStaging(const PinnedStaging&) = delete;
+  PinnedStaging& operator=(const PinnedStaging&) = delete;
+  // a slot of `bytes`, or nullptr when the rest of the buffer is too small
+  void* take(std::size_t bytes);
+  void reset() { used_ = 0; }
+  std::size_t capacity() const { return cap_; }
+
+ private:
+  char* base_ = nullptr;
+  std::size_t cap_ = 0, used_ = 0;
+};
+
 // Grow-only device buffer (src/compute_cuda.cu:72-79 reserve semantics).
 template <class T>
 struct DevBuf {
@@ -91,6 +111,8 @@ class Context {
   // queued on the main stream so far, join() makes the main stream wait for
   // everything queued on it.
   hipStream_t aux_stream() const { return aux_; }
+  // pinned staging of the main stream's control transfers (allocated once)
+  PinnedStaging& staging();
   void fork();
   void join();
   rocblas_handle blas() const { return blas_; }
@@ -145,6 +167,7 @@ class Context {
   hipStream_t stream_{};
   hipStream_t aux_{};
   hipEvent_t fork_event_{}, join_event_{};
+  std::unique_ptr<PinnedStaging> staging_;
   rocblas_handle blas_{};
   std::string name_;
   DevBuf<double> work_;
