@@ -414,20 +414,15 @@ bool Solver::cholesky_normalise(const double* v, const double* metric, const dou
                                 bool exact) {
   int* info = reinterpret_cast<int*>(small_.p);
   auto* U = reinterpret_cast<rocblas_double_complex*>(gh);
+  // everything is queued first and checked after one synchronisation: a
+  // failed factorisation only leaves garbage in columns that are not kept
   check(rocsolver_zpotrf(ctx_.blas(), rocblas_fill_upper, rocblas_int(cols), U, rocblas_int(cols), info),
         "rocsolver_zpotrf");
   double* diag = small_.p + 8;
   check(launch_diagonals(gram, cols, gh, cols, cols, diag, s_), "diagonals");
-  const std::vector<double> h = to_host(small_.p, 8 + 2 * cols);
-  if (*reinterpret_cast<const int*>(h.data())) return false;
-  double gmax = 0, umin = std::numeric_limits<double>::infinity();
-  for (long i = 0; i < cols; ++i) {
-    gmax = std::max(gmax, h[8 + i]);
-    umin = std::min(umin, h[8 + cols + i] * h[8 + cols + i]);
-  }
-  if (!(gmax > 0) || !(umin >= 1e-8 * gmax) || !std::isfinite(gmax)) return false;
+  int* trtri_info = reinterpret_cast<int*>(small_.p + 1);
   check(rocsolver_ztrtri(ctx_.blas(), rocblas_fill_upper, rocblas_diagonal_non_unit, rocblas_int(cols), U,
-                         rocblas_int(cols), info),
+                         rocblas_int(cols), trtri_info),
         "rocsolver_ztrtri");
   check(launch_zero_lower(gh, cols, cols, s_), "zero lower");
   double* w = col(basis_, size_, n_);
@@ -436,7 +431,19 @@ bool Solver::cholesky_normalise(const double* v, const double* metric, const dou
   gemm(exact, 0, 0, n_, cols, cols, 1.0, metric, n_, gh, cols, sw, n_);
   double* g2 = m2_.p;
   gemm(exact, 1, 0, cols, cols, n_, 1.0, w, n_, sw, n_, g2, cols);
-  const double defect = identity_defect(g2, cols, cols, nullptr);
+  auto* bits = reinterpret_cast<unsigned long long*>(small_.p + 2);
+  check(hipMemsetAsync(bits, 0, sizeof(unsigned long long), s_), "memset");
+  check(launch_identity_defect(g2, cols, cols, nullptr, cols, bits, s_), "identity defect");
+  const std::vector<double> h = to_host(small_.p, 8 + 2 * cols);
+  if (*reinterpret_cast<const int*>(h.data())) return false;
+  double gmax = 0, umin = std::numeric_limits<double>::infinity();
+  for (long i = 0; i < cols; ++i) {
+    gmax = std::max(gmax, h[8 + i]);
+    umin = std::min(umin, h[8 + cols + i] * h[8 + cols + i]);
+  }
+  if (!(gmax > 0) || !(umin >= 1e-8 * gmax) || !std::isfinite(gmax)) return false;
+  if (*reinterpret_cast<const int*>(h.data() + 1)) return false;
+  const double defect = h[2];  // the bits of a non-negative double (or NaN)
   if (!(defect <= (exact ? 1e-10 : 1e-4))) return false;
   ++stats_.cholesky_blocks;
   return true;
