@@ -205,6 +205,11 @@ def main():
         if k in products:
             kernels[k]["real_products_per_complex_mac"] = products[k]
             kernels[k]["mfma_executed_frac"] = kernels[k]["frac"] * products[k] / 4
+        else:
+            # inside the solve the local-H pipeline runs on a second stream beside
+            # the projector GEMMs, so its event time includes the CUs it shares;
+            # isolated_local_h times it alone
+            kernels[k]["concurrent_with"] = "projector GEMM chain on the main stream"
     roofline = {"bound": bound, "achieved": kernels[dom]["achieved"], "peak": peak, "unit": unit,
                 "frac": kernels[dom]["frac"], "traffic": traffic, "kernel": names[dom],
                 "work_per_launch": kernels[dom]["work_per_launch"],
