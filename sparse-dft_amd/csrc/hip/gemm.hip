@@ -359,13 +359,18 @@ __global__ void k_atom_blocks_f32(int np, int columns, int atoms, const int* __r
   }
 }
 
-// Tile configurations: 0 = 64 x 64 with prefetch, 1 = 64 x 64 without
-// (FP64 split-K slabs, where the measured prefetch variant is slower),
-// 2 = 128 x 32 with prefetch (tall, narrow outputs: rotations, back-projections).
+// Tile configurations: 0 = 64 x 64 with prefetch (FP32), 1 = 64 x 64 without
+// (FP32 split-K slabs), 2 = 128 x 32 with prefetch (tall, narrow outputs:
+// rotations), 3 = 64 x 32 with prefetch (every other FP64 shape). Measured on
+// the Si64 FP64 shapes against the 64 x 64 / 128 x 32 plans: slabs V^H W
+// 800 x 160 x n 51.2 against 39.8 TFLOP/s, projections 512 x 160 x n 62.6 / 55.0,
+// back-projections n x 160 x 512 54.6 / 42.2, Gram 160 x 160 x n 38.9 / 30.0,
+// Rayleigh-Ritz 800^3 47.2 / 37.0; rotations n x 160 x 800 stay at 128 x 32
+// (65.5 against 62.6).
 struct Tile {
   int bm, bn;
 };
-constexpr Tile kTiles[3] = {{64, 64}, {64, 64}, {128, 32}};
+constexpr Tile kTiles[4] = {{64, 64}, {64, 64}, {128, 32}, {64, 32}};
 
 bool tall_shape(int M, int N) { return M >= 2048 && N <= 192 && long(M) >= 8L * N; }
 
@@ -399,6 +404,8 @@ hipError_t dispatch_ops(int cfg, int aop, int bop, dim3 grid, int M, int N, int 
   }
   hipError_t e;
   if (cfg == 2) e = launch_cfg<T, TA, TB, 2>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work, s);
+  else if (cfg == 3 && sizeof(T) == 8)
+    e = launch_cfg<T, TA, TB, 3>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work, s);
   else if (cfg == 1 && sizeof(T) == 8)
     e = launch_cfg<T, TA, TB, 1>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work, s);
   else e = launch_cfg<T, TA, TB, 0>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work, s);
@@ -411,17 +418,35 @@ struct Plan {
   int cfg, splits, kps;
   dim3 grid;
 };
-Plan plan_for(bool fp64, int M, int N, int K, bool have_work) {
+// splits along K: double while the workgroups number fewer than `target` and
+// every split keeps >= 256 of K
+int splits_for(long tiles, int K, long target) {
+  int splits = 1;
+  while (tiles * splits < target && long(K) / (2 * splits) >= 256 && splits < 64) splits *= 2;
+  return splits;
+}
+long tile_count(int M, int N, int cfg) {
+  return long((M + kTiles[cfg].bm - 1) / kTiles[cfg].bm) * ((N + kTiles[cfg].bn - 1) / kTiles[cfg].bn);
+}
+// FP64: 64 x 32 tiles and >= 2048 workgroups (tall shapes need no split);
+// rotations (tall, normal A) 128 x 32
+int fp64_cfg(int M, int N, int aop) { return tall_shape(M, N) && !aop ? 2 : 3; }
+int fp64_splits(int M, int N, int K, int aop) {
+  const int cfg = fp64_cfg(M, N, aop);
+  return splits_for(tile_count(M, N, cfg), K, tall_shape(M, N) ? 512 : 2048);
+}
+Plan plan_for(bool fp64, int aop, int M, int N, int K, bool have_work) {
   Plan p{};
   const bool tall = tall_shape(M, N);
-  const int bm = tall ? 128 : 64, bn = tall ? 32 : 64;
-  p.splits = zgemm_splits(M, N, K);
+  const int cfg64 = fp64_cfg(M, N, aop);
+  p.splits = fp64 ? fp64_splits(M, N, K, aop) : splits_for(tile_count(M, N, tall ? 2 : 0), K, 512);
   if (p.splits > 1 && !have_work) p.splits = 1;
   p.kps = (K + p.splits - 1) / p.splits;
   p.kps = ((p.kps + BK - 1) / BK) * BK;
   p.splits = (K + p.kps - 1) / p.kps;
-  p.cfg = tall ? 2 : (fp64 && p.splits > 1 ? 1 : 0);
-  p.grid = dim3((M + bm - 1) / bm, (N + bn - 1) / bn, p.splits);
+  p.cfg = fp64 ? cfg64 : (tall ? 2 : 0);
+  p.grid = dim3((M + kTiles[p.cfg].bm - 1) / kTiles[p.cfg].bm, (N + kTiles[p.cfg].bn - 1) / kTiles[p.cfg].bn,
+                p.splits);
   return p;
 }
 }  // namespace
@@ -433,12 +458,11 @@ std::size_t zgemm_workspace_bytes(int M, int N, int K) {
   return splits > 1 ? std::size_t(splits) * M * N * 16 : 0;
 }
 
+// the largest split count any plan of the shape uses (workspace sizing)
 int zgemm_splits(int M, int N, int K) {
   const bool tall = tall_shape(M, N);
-  const long tiles = long((M + (tall ? 128 : 64) - 1) / (tall ? 128 : 64)) * ((N + (tall ? 32 : 64) - 1) / (tall ? 32 : 64));
-  // aim for >= 512 workgroups on 256 CUs while keeping >= 256 K per split
-  int splits = 1;
-  while (tiles * splits < 512 && long(K) / (2 * splits) >= 256 && splits < 64) splits *= 2;
+  int splits = splits_for(tile_count(M, N, tall ? 2 : 0), K, 512);
+  for (int aop = 0; aop < 2; ++aop) splits = std::max(splits, fp64_splits(M, N, K, aop));
   return splits;
 }
 
@@ -451,7 +475,7 @@ hipError_t zgemm(Precision prec, int aop, int bop, int M, int N, int K, double a
     return launch_zaxpby(M, N, 0.0, nullptr, 0, beta, D, ldd, C, ldc, s);
   }
   const bool room = work != nullptr && work_bytes >= zgemm_workspace_bytes(M, N, K);
-  const Plan pl = plan_for(prec == Precision::FP64, M, N, K, room);
+  const Plan pl = plan_for(prec == Precision::FP64, aop, M, N, K, room);
   double* wk = pl.splits > 1 ? work : nullptr;
   hipError_t e;
   if (prec == Precision::FP64)
@@ -470,7 +494,7 @@ hipError_t zgemm_f32_storage(int aop, int bop, int M, int N, int K, double alpha
                              double* work, std::size_t work_bytes, hipStream_t s, const LaunchEvents* ev) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
   const bool room = work != nullptr && work_bytes >= zgemm_workspace_bytes(M, N, K);
-  const Plan pl = plan_for(false, M, N, K, room);
+  const Plan pl = plan_for(false, aop, M, N, K, room);
   double* wk = pl.splits > 1 ? work : nullptr;
   hipError_t e = dispatch_ops<float, float, double>(pl.cfg, aop, bop, pl.grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd,
                                                     alpha, beta, pl.kps, wk, s, ev);
