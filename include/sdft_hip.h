@@ -257,6 +257,15 @@ int sdft_hip_hermitian_eigen(sdft_hip_context* ctx, size_t n, const double* a, s
                                            iteration, 2 = one-launch tridiagonalisation and divide and
                                            conquer, 0 = zheevd */);
 
+/* Davidson seed block of the device solver (no counterpart as a function in
+ * the reference; src/pw_core.cpp:1027-1112 builds it inline; exposed for
+ * tests): for `roots` roots at `tolerance`, the warm columns (initial, n x
+ * roots, perturbed) when 2 roots < n and initial is not NULL, then 2 roots
+ * guard columns; block (host, n x (3 roots) complex at most) receives
+ * *columns columns. */
+int sdft_hip_davidson_seeds(sdft_hip_operator* op, size_t roots, double tolerance, const double* initial,
+                            double* block, size_t* columns);
+
 /* Symmetric tridiagonal stage of that eigensolver (no counterpart in the
  * reference; exposed for tests): all n eigenvalues of T (diagonal d, off-
  * diagonal e of length n - 1) by Sturm-count multisection, ascending to within

@@ -317,6 +317,63 @@ dim3 grid_cols(long n, long cols) {
 }
 }  // namespace
 
+// Davidson seed block (src/pw_core.cpp:1027-1112) on the device, bit-equal to
+// the sequential host recipe: uniform() call c returns splitmix64 of
+// state0 + (c + 1) * golden, so every call is computed where it is needed;
+// column c of the block consumes calls [2 n c, 2 n (c + 1)) (its broad()
+// vector b). The two norms are summed by single threads in index order, as
+// the host loops do (this file is built without FMA contraction).
+//   warm column (c < roots when warm): src + (pert |src|) b / |b|
+//   guard column g: 0.05 b / |b|, plus 1 at row order[g]
+__device__ inline double seed_uniform(unsigned long long c) {
+  unsigned long long z = 0x6a09e667f3bcc909ULL + (c + 1) * 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  z ^= z >> 31;
+  return static_cast<double>(z >> 11) * 0x1.0p-53 - .5;
+}
+__global__ __launch_bounds__(TPB) void k_davidson_seeds(long n, long warm_cols, const double* __restrict__ init,
+                                                        long ldi, double pert, const int* __restrict__ order,
+                                                        double* __restrict__ out, long ldo) {
+  __shared__ double norms[2];
+  const long c = blockIdx.x;
+  const bool warm = c < warm_cols;
+  double* col = out + 2 * c * ldo;
+  const double* src = warm ? init + 2 * c * ldi : nullptr;
+  const unsigned long long base = static_cast<unsigned long long>(c) * 2ull * static_cast<unsigned long long>(n);
+  for (long i = threadIdx.x; i < n; i += TPB) {
+    col[2 * i] = seed_uniform(base + 2 * i);
+    col[2 * i + 1] = seed_uniform(base + 2 * i + 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double nn = 0.;
+    for (long i = 0; i < n; ++i) nn += col[2 * i] * col[2 * i] + col[2 * i + 1] * col[2 * i + 1];
+    norms[0] = sqrt(nn);
+  } else if (threadIdx.x == 64 && warm) {
+    double nn = 0.;
+    for (long i = 0; i < n; ++i) nn += src[2 * i] * src[2 * i] + src[2 * i + 1] * src[2 * i + 1];
+    norms[1] = sqrt(nn);
+  }
+  __syncthreads();
+  const double nb = norms[0];
+  const double s = warm ? pert * norms[1] : 0.;
+  for (long i = threadIdx.x; i < n; i += TPB) {
+    const double br = col[2 * i] / nb, bi = col[2 * i + 1] / nb;
+    if (warm) {
+      col[2 * i] = src[2 * i] + s * br;
+      col[2 * i + 1] = src[2 * i + 1] + s * bi;
+    } else {
+      col[2 * i] = .05 * br;
+      col[2 * i + 1] = .05 * bi;
+    }
+  }
+  if (!warm) {
+    __syncthreads();
+    if (threadIdx.x == 0) col[2 * static_cast<long>(order[c - warm_cols])] += 1.;
+  }
+}
+
 #define SDFT_CHECKED(call) \
   do {                     \
     call;                  \
@@ -337,6 +394,12 @@ hipError_t launch_complex_dots(const double* a, long lda, long ca, const double*
 hipError_t launch_col_norms(const double* x, long ldx, long n, long cols, double* out, hipStream_t s) {
   if (cols <= 0) return hipSuccess;
   SDFT_CHECKED(hipLaunchKernelGGL(k_col_norms, dim3(unsigned(cols)), dim3(TPB), 0, s, x, ldx, n, out));
+}
+hipError_t launch_davidson_seeds(long n, long warm_cols, const double* init, long ldi, double pert, long guards,
+                                 const int* order, double* out, long ldo, hipStream_t s) {
+  if (n <= 0 || warm_cols + guards <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_davidson_seeds, dim3(unsigned(warm_cols + guards)), dim3(TPB), 0, s, n, warm_cols,
+                                  init, ldi, pert, order, out, ldo));
 }
 hipError_t launch_kinetic_weights(const double* x, long ldx, long n, long cols, const double* kinetic, double* out,
                                   hipStream_t s) {

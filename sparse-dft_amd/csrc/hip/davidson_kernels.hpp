@@ -11,6 +11,12 @@ hipError_t launch_real_dots(const double* a, long lda, const double* b, long ldb
 hipError_t launch_complex_dots(const double* a, long lda, long ca, const double* b, long ldb, long cb, long n,
                                double* out, long ldo, hipStream_t s);
 hipError_t launch_col_norms(const double* x, long ldx, long n, long cols, double* out, hipStream_t s);
+// Davidson seed block, n x (warm_cols + guards) complex (ld ldo): warm columns
+// from init (n x warm_cols, ld ldi) perturbed by pert |init_j| times the
+// normalised splitmix64 vector of the column, then guard columns 0.05 b / |b| + e_order[g]
+// (src/pw_core.cpp:1027-1112; the stream of uniform() calls of the host recipe).
+hipError_t launch_davidson_seeds(long n, long warm_cols, const double* init, long ldi, double pert, long guards,
+                                 const int* order, double* out, long ldo, hipStream_t s);
 hipError_t launch_kinetic_weights(const double* x, long ldx, long n, long cols, const double* kinetic, double* out,
                                   hipStream_t s);
 hipError_t launch_precondition(const double* r, long ldr, const int* src, long cols, const double* wk,

@@ -684,6 +684,29 @@ int sdft_hip_hermitian_eigen(sdft_hip_context* ctx, size_t n, const double* a, s
   });
 }
 
+int sdft_hip_davidson_seeds(sdft_hip_operator* op, size_t roots, double tolerance, const double* initial,
+                            double* block, size_t* columns) {
+  return guarded([&] {
+    if (!op || !roots || !block || !columns || !(tolerance > 0)) throw std::invalid_argument("HIP: invalid seed arguments");
+    auto& dev = op->impl;
+    const long n = long(dev.n()), R = long(roots);
+    if (R > n) throw std::invalid_argument("HIP: more roots than basis functions");
+    hipStream_t s = dev.context().stream();
+    const bool warm = initial != nullptr && 2 * R < n;
+    const long cols = sdft_hip::davidson_seed_columns(n, R, warm);
+    sdft_hip::DevBuf<double> init, out;
+    if (warm) {
+      init.reserve(2 * std::size_t(n) * R);
+      sdft_hip::check(hipMemcpyAsync(init.p, initial, 16 * std::size_t(n) * R, hipMemcpyHostToDevice, s), "upload");
+    }
+    out.reserve(2 * std::size_t(n) * cols);
+    sdft_hip::davidson_seed_block(dev, R, tolerance, warm ? init.p : nullptr, out.p, s);
+    sdft_hip::check(hipMemcpyAsync(block, out.p, 16 * std::size_t(n) * cols, hipMemcpyDeviceToHost, s), "download");
+    dev.context().sync();
+    *columns = std::size_t(cols);
+  });
+}
+
 int sdft_hip_tridiagonal_eigen(sdft_hip_context* ctx, size_t n, const double* d, const double* e, size_t k,
                                double* values, double* vectors, int* status) {
   return guarded([&] {

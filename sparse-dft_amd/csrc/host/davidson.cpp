@@ -626,57 +626,18 @@ DavidsonOutput Solver::run(const C* initial) {
   bool strict_projection = false;
   unsigned polish_attempts = 0, manifold_attempts = 0;
   double strict_entry = std::numeric_limits<double>::infinity();
-  const std::vector<double>& hd = op_.h_diagonal();
-  const std::vector<double>& sd = op_.s_diagonal();
-  std::vector<long> order(n);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](long i, long j) { return hd[i] / sd[i] < hd[j] / sd[j]; });
-  // seeds (src/pw_core.cpp:1027-1112): splitmix64 broad seeds, warm start +
-  // clamp(100 tol, 1e-10, 1e-6) perturbation, 2 roots canonical guards
-  std::uint64_t state = 0x6a09e667f3bcc909ULL;
-  auto uniform = [&]() {
-    std::uint64_t z = (state += 0x9e3779b97f4a7c15ULL);
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-    z ^= z >> 31;
-    return static_cast<double>(z >> 11) * 0x1.0p-53 - .5;
-  };
-  auto broad = [&](std::vector<C>& s) {
-    s.resize(n);
-    double nn = 0;
-    for (long i = 0; i < n; ++i) {
-      const double re = uniform();
-      const double im = uniform();
-      s[i] = C(re, im);
-      nn += std::norm(s[i]);
-    }
-    nn = std::sqrt(nn);
-    for (auto& z : s) z /= nn;
-  };
-  const double warm_pert = std::clamp(100. * tol_, 1e-10, 1e-6);
+  const std::vector<long> order = davidson_order(op_);
   const bool warm = initial != nullptr && 2 * R < n;
-  const long guards = std::min(n, 2 * R);
-  const long nseeds = (warm ? R : 0) + guards;
-  std::vector<C> seeds(std::size_t(n) * nseeds);
-  std::vector<C> b;
-  long sc = 0;
+  const long nseeds = davidson_seed_columns(n, R, warm);
   if (warm)
-    for (long j = 0; j < R; ++j, ++sc) {
-      const C* src = initial + std::size_t(j) * n;
-      double nn = 0;
-      for (long i = 0; i < n; ++i) nn += std::norm(src[i]);
-      nn = std::sqrt(nn);
-      broad(b);
-      for (long i = 0; i < n; ++i) seeds[std::size_t(sc) * n + i] = src[i] + warm_pert * nn * b[i];
-    }
-  for (long j = 0; j < guards; ++j, ++sc) {
-    broad(b);
-    for (long i = 0; i < n; ++i) seeds[std::size_t(sc) * n + i] = .05 * b[i];
-    seeds[std::size_t(sc) * n + order[j]] += 1.;
-  }
-  DevBuf<double> block;
-  block.upload(reinterpret_cast<const double*>(seeds.data()), 2 * seeds.size(), s_);
-  append_block(block.p, nseeds);
+    check(hipMemcpyAsync(corr_.p, initial, 16 * std::size_t(n) * R, hipMemcpyHostToDevice, s_), "initial block");
+  // the retained block holds cap_ columns; a local subspace of fewer than
+  // 3 roots (band groups) takes a block of its own
+  DevBuf<double> wide;
+  double* block = retained_.p;
+  if (nseeds > cap_) block = wide.reserve(2 * std::size_t(n) * nseeds);
+  davidson_seed_block(op_, R, tol_, warm ? corr_.p : nullptr, block, s_);
+  append_block(block, nseeds);
 
   values_.assign(R, 0.);
   res_norms_.assign(R, 0.);
@@ -848,6 +809,40 @@ DavidsonOutput Solver::run(const C* initial) {
   return out;
 }
 }  // namespace
+
+std::vector<long> davidson_order(const DeviceOperator& op) {
+  const std::vector<double>& hd = op.h_diagonal();
+  const std::vector<double>& sd = op.s_diagonal();
+  std::vector<long> order(hd.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](long i, long j) { return hd[i] / sd[i] < hd[j] / sd[j]; });
+  return order;
+}
+
+long davidson_seed_columns(long n, long roots, bool warm) { return (warm ? roots : 0) + std::min(n, 2 * roots); }
+
+// seeds (src/pw_core.cpp:1027-1112): splitmix64 broad seeds, warm start +
+// clamp(100 tol, 1e-10, 1e-6) perturbation, 2 roots canonical guards; the
+// block is generated on the device (launch_davidson_seeds) from the same
+// stream of uniform() calls, in the host recipe's order
+long davidson_seed_block(DeviceOperator& op, long roots, double tol, const double* initial_device, double* block,
+                         hipStream_t s) {
+  const long n = long(op.n());
+  const bool warm = initial_device != nullptr && 2 * roots < n;
+  const long guards = std::min(n, 2 * roots);
+  const std::vector<long> order = davidson_order(op);
+  std::vector<int> guard_rows(order.begin(), order.begin() + guards);
+  DevBuf<int> guard_dev;
+  guard_dev.reserve(std::size_t(guards));
+  check(hipMemcpyAsync(guard_dev.p, guard_rows.data(), sizeof(int) * std::size_t(guards), hipMemcpyHostToDevice, s),
+        "guard rows");
+  check(launch_davidson_seeds(n, warm ? roots : 0, initial_device, n, std::clamp(100. * tol, 1e-10, 1e-6), guards,
+                              guard_dev.p, block, n, s),
+        "seeds");
+  // guard_dev is released on return
+  check(hipStreamSynchronize(s), "seeds");
+  return davidson_seed_columns(n, roots, warm);
+}
 
 DavidsonOutput davidson(DeviceOperator& op, const DavidsonParams& params, const std::complex<double>* initial) {
   if (params.band_group_size > 0 && !params.band_group_local) return davidson_band_groups(op, params, initial);

@@ -55,6 +55,15 @@ struct DavidsonOutput {
 
 DavidsonOutput davidson(DeviceOperator& op, const DavidsonParams& params,
                         const std::complex<double>* initial /* n x roots or nullptr */);
+
+// Canonical order of the basis by H/S diagonal (src/pw_core.cpp:1016-1021).
+std::vector<long> davidson_order(const DeviceOperator& op);
+// The Davidson seed block of `roots` roots at tolerance `tol` (warm start
+// from `initial_device`, n x roots on the device, or none), generated on the
+// device into `block` (n x columns, ld n); returns the column count.
+long davidson_seed_block(DeviceOperator& op, long roots, double tol, const double* initial_device, double* block,
+                         hipStream_t s);
+long davidson_seed_columns(long n, long roots, bool warm);
 // The bounded band-group solver: local Davidson solves of band_group_size
 // roots with the completed bands excluded, then a global roots x roots
 // Rayleigh-Ritz rotation certified on fresh FP64 H and S (two sweeps).

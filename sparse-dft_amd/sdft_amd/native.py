@@ -406,6 +406,16 @@ class Operator:
         self.np = npj.value
         self.sticks = sticks.value
 
+    def davidson_seeds(self, roots, tolerance=1e-12, initial=None):
+        """The device Davidson seed block (n x columns complex) for `roots` roots."""
+        cols_max = 3 * int(roots)
+        block = np.zeros((self.n, cols_max), dtype=np.complex128, order="F")
+        init = None if initial is None else np.asfortranarray(initial[:, :roots], dtype=np.complex128)
+        cols = ct.c_size_t(0)
+        _check(lib().sdft_hip_davidson_seeds(self.handle, ct.c_size_t(roots), ct.c_double(tolerance),
+                                             _p(init) if init is not None else None, _p(block), ct.byref(cols)))
+        return block[:, :cols.value]
+
     def close(self):
         if self.handle:
             lib().sdft_hip_operator_destroy(self.handle)

@@ -84,3 +84,61 @@ def test_davidson_invalid_arguments(si8_dev):
         si8_dev.davidson(0)
     with pytest.raises(ValueError):
         si8_dev.davidson(4, subspace_factor=3)
+
+
+def _reference_seeds(n, roots, tol, initial, order):
+    """src/pw_core.cpp:1027-1112 seeding, restated with the host recipe's
+    sequential sums (splitmix64 in exact integer arithmetic)."""
+    import math
+    mask = (1 << 64) - 1
+    state = 0x6A09E667F3BCC909
+
+    def uniform():
+        nonlocal state
+        state = (state + 0x9E3779B97F4A7C15) & mask
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        z ^= z >> 31
+        return float(z >> 11) * 2.0 ** -53 - 0.5
+
+    def broad():
+        b = [complex(uniform(), uniform()) for _ in range(n)]
+        nn = 0.0
+        for z in b:
+            nn += z.real * z.real + z.imag * z.imag
+        nn = math.sqrt(nn)
+        return [complex(z.real / nn, z.imag / nn) for z in b]
+
+    pert = min(max(100.0 * tol, 1e-10), 1e-6)
+    warm = initial is not None and 2 * roots < n
+    cols = []
+    if warm:
+        for j in range(roots):
+            src = initial[:, j]
+            nn = 0.0
+            for z in src:
+                nn += z.real * z.real + z.imag * z.imag
+            s = pert * math.sqrt(nn)
+            b = broad()
+            cols.append([complex(src[i].real + s * b[i].real, src[i].imag + s * b[i].imag) for i in range(n)])
+    for j in range(min(n, 2 * roots)):
+        b = broad()
+        col = [complex(0.05 * z.real, 0.05 * z.imag) for z in b]
+        col[order[j]] += 1.0
+        cols.append(col)
+    return np.array(cols, dtype=np.complex128).T
+
+
+@pytest.mark.parametrize("warm", [True, False])
+def test_device_seeds_equal_reference_recipe(si8, si8_dev, warm):
+    """The device seed block (generated in parallel from the splitmix64
+    stream) equals the reference's sequential recipe bit for bit."""
+    roots = 16
+    init = syn.initial_psi(si8, roots, 0) if warm else None
+    dev = si8_dev.davidson_seeds(roots, 1e-12, init)
+    info = si8_dev.info()
+    order = np.argsort(info["h_diag"] / info["s_diag"], kind="stable")
+    ref = _reference_seeds(si8_dev.n, roots, 1e-12, init, order)
+    assert dev.shape == ref.shape
+    assert np.array_equal(dev, ref)
