@@ -517,12 +517,12 @@ int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* par
     p.verbose = params->verbose;
     p.timing = params->timing;
     p.band_group_size = params->band_group_size;
+    p.vectors_out = reinterpret_cast<std::complex<double>*>(result->eigenvectors);
     if (p.search_fp32 != 0 && p.search_fp32 != 1) throw std::invalid_argument("HIP: search precision must be 0 or 1");
     const auto out = sdft_hip::davidson(op->impl, p, reinterpret_cast<const std::complex<double>*>(initial));
     std::copy(out.eigenvalues.begin(), out.eigenvalues.end(), result->eigenvalues);
     std::copy(out.residual_norms.begin(), out.residual_norms.end(), result->residual_norms);
-    if (result->eigenvectors)
-      std::memcpy(result->eigenvectors, out.eigenvectors.data(), out.eigenvectors.size() * sizeof(std::complex<double>));
+    // eigenvectors went straight to result->eigenvectors (p.vectors_out)
     result->iterations = out.stats.iterations;
     result->converged = out.stats.converged ? 1 : 0;
     result->h_columns_search = out.stats.h_columns_search;

@@ -798,8 +798,12 @@ DavidsonOutput Solver::run(const C* initial) {
   DavidsonOutput out;
   out.eigenvalues = values_;
   out.residual_norms = res_norms_;
-  out.eigenvectors.resize(std::size_t(n) * R);
-  check(hipMemcpyAsync(out.eigenvectors.data(), x_.p, 16 * std::size_t(n) * R, hipMemcpyDeviceToHost, s_), "vectors");
+  std::complex<double>* dst = p_.vectors_out;
+  if (!dst) {
+    out.eigenvectors.resize(std::size_t(n) * R);
+    dst = out.eigenvectors.data();
+  }
+  check(hipMemcpyAsync(dst, x_.p, 16 * std::size_t(n) * R, hipMemcpyDeviceToHost, s_), "vectors");
   sync();
   stats_.iterations = iterations;
   stats_.rr_stedc = eig_ ? eig_->stedc_solves() : 0;
@@ -907,6 +911,7 @@ DavidsonOutput davidson_band_groups(DeviceOperator& op, const DavidsonParams& pa
       DavidsonParams lp = params;
       lp.roots = count;
       lp.band_group_local = 1;
+      lp.vectors_out = nullptr;
       lp.capacity = std::min(n, 5 * G);
       if (first) {
         // S-orthonormalise the completed bands (Cholesky), src/band_group_solver.cpp:100-123
@@ -1002,8 +1007,12 @@ DavidsonOutput davidson_band_groups(DeviceOperator& op, const DavidsonParams& pa
     converged = true;
     for (double r : out.residual_norms) converged = converged && r <= params.tolerance;
   }
-  out.eigenvectors.resize(std::size_t(n) * R);
-  check(hipMemcpyAsync(out.eigenvectors.data(), x.p, 16 * std::size_t(n) * R, hipMemcpyDeviceToHost, s), "vectors");
+  std::complex<double>* dst = params.vectors_out;
+  if (!dst) {
+    out.eigenvectors.resize(std::size_t(n) * R);
+    dst = out.eigenvectors.data();
+  }
+  check(hipMemcpyAsync(dst, x.p, 16 * std::size_t(n) * R, hipMemcpyDeviceToHost, s), "vectors");
   sync();
   out.stats.converged = converged;
   out.stats.seconds_total = now() - t_start;

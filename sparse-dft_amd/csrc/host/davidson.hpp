@@ -33,6 +33,9 @@ struct DavidsonParams {
   const double* excluded = nullptr;
   long excluded_count = 0;
   int band_group_local = 0;
+  // host n x roots destination of the eigenvectors: copied there directly
+  // (DavidsonOutput::eigenvectors stays empty); nullptr keeps them in the output
+  std::complex<double>* vectors_out = nullptr;
 };
 
 struct DavidsonStats {
