@@ -20,10 +20,36 @@ namespace {
 using C = std::complex<double>;
 double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
+// Device buffers of a solve; the operator keeps one set between solves
+// (DevBuf grows on demand), so repeated solves allocate nothing.
+struct Workspace {
+  DevBuf<double> basis, hb, sb, ch, cs, x, sx, hx, res;
+  DevBuf<double> small, w1, w2, w3, w4, m1, m2, m3, rr, rrb, qbuf, corr, retained;
+  std::unique_ptr<HermitianEigensolver> eig;
+  long eig_order = 0;
+  bool busy = false;
+};
+
+std::shared_ptr<Workspace> acquire_workspace(DeviceOperator& op) {
+  std::shared_ptr<void>& slot = op.davidson_cache();
+  auto ws = std::static_pointer_cast<Workspace>(slot);
+  if (!ws || ws->busy) {
+    // a solve already holds the operator's set: this one gets its own
+    ws = std::make_shared<Workspace>();
+    if (!slot) slot = ws;
+  }
+  ws->busy = true;
+  return ws;
+}
+
 class Solver {
  public:
   Solver(DeviceOperator& op, const DavidsonParams& p)
-      : op_(op), ctx_(op.context()), s_(ctx_.stream()), p_(p), n_(long(op.n())), R_(p.roots), tol_(p.tolerance) {
+      : op_(op), ctx_(op.context()), s_(ctx_.stream()), p_(p), n_(long(op.n())), R_(p.roots), tol_(p.tolerance),
+        ws_(acquire_workspace(op)), basis_(ws_->basis), hb_(ws_->hb), sb_(ws_->sb), ch_(ws_->ch), cs_(ws_->cs),
+        x_(ws_->x), sx_(ws_->sx), hx_(ws_->hx), res_(ws_->res), small_(ws_->small), w1_(ws_->w1), w2_(ws_->w2),
+        w3_(ws_->w3), w4_(ws_->w4), m1_(ws_->m1), m2_(ws_->m2), m3_(ws_->m3), rr_(ws_->rr), rrb_(ws_->rrb),
+        qbuf_(ws_->qbuf), corr_(ws_->corr), retained_(ws_->retained) {
     if (!n_ || R_ <= 0 || R_ > n_ || p.max_iterations <= 0 || !std::isfinite(tol_) || tol_ <= 0)
       throw std::invalid_argument("HIP: invalid matrix-free Davidson problem");
     if (p.subspace_factor < 4 || p.subspace_factor > 16)
@@ -60,12 +86,24 @@ class Solver {
     qbuf_.reserve(2 * cap_ * cap_ + 16);
     corr_.reserve(2 * n_ * R_);
     retained_.reserve(2 * n_ * cap_);
-    eig_ = std::make_unique<HermitianEigensolver>(ctx_, cap_);
+    if (!ws_->eig || ws_->eig_order < cap_) {
+      ws_->eig = std::make_unique<HermitianEigensolver>(ctx_, cap_);
+      ws_->eig_order = cap_;
+    }
+    eig_ = ws_->eig.get();
+    stedc_base_ = eig_->stedc_solves();
     if (p.excluded_count < 0 || (p.excluded_count && !p.excluded))
       throw std::invalid_argument("invalid completed-band dimensions");
     excl_ = p.excluded_count;
     if (excl_) setup_excluded(p.excluded);
   }
+  ~Solver() {
+    // queued work may still use the buffers; the next solve on this stream
+    // is ordered after it
+    ws_->busy = false;
+  }
+  Solver(const Solver&) = delete;
+  Solver& operator=(const Solver&) = delete;
   DavidsonOutput run(const C* initial);
 
  private:
@@ -268,9 +306,11 @@ class Solver {
   long n_, R_, cap_ = 0;
   double tol_;
   bool exact_ = true, incremental_ = true, converged_ = false;
-  DevBuf<double> basis_, hb_, sb_, ch_, cs_, x_, sx_, hx_, res_;
-  DevBuf<double> small_, w1_, w2_, w3_, w4_, m1_, m2_, m3_, rr_, rrb_, qbuf_, corr_, retained_;
-  std::unique_ptr<HermitianEigensolver> eig_;
+  std::shared_ptr<Workspace> ws_;
+  DevBuf<double>&basis_, &hb_, &sb_, &ch_, &cs_, &x_, &sx_, &hx_, &res_;
+  DevBuf<double>&small_, &w1_, &w2_, &w3_, &w4_, &m1_, &m2_, &m3_, &rr_, &rrb_, &qbuf_, &corr_, &retained_;
+  HermitianEigensolver* eig_ = nullptr;
+  long stedc_base_ = 0;  // divide-and-conquer solves of the shared eigensolver before this solve
   PinnedStaging& stage_ = ctx_.staging();
   long excl_ = 0;
   const double* excluded_ = nullptr;
@@ -806,7 +846,7 @@ DavidsonOutput Solver::run(const C* initial) {
   check(hipMemcpyAsync(dst, x_.p, 16 * std::size_t(n) * R, hipMemcpyDeviceToHost, s_), "vectors");
   sync();
   stats_.iterations = iterations;
-  stats_.rr_stedc = eig_ ? eig_->stedc_solves() : 0;
+  stats_.rr_stedc = eig_ ? eig_->stedc_solves() - stedc_base_ : 0;
   stats_.converged = converged_;
   stats_.seconds_total = now() - t_start;
   out.stats = stats_;

@@ -258,6 +258,9 @@ class DeviceOperator {
   std::vector<double> local_real_natural();
   int nsticks() const { return grid_.dev.nsticks; }
   Context& context() { return ctx_; }
+  // device buffers of the Davidson solver, kept between solves of this
+  // operator (type known to davidson.cpp only)
+  std::shared_ptr<void>& davidson_cache() { return davidson_cache_; }
   std::size_t max_chunk() const { return chunk_; }
   const GridPlanDev& grid_plan() const { return grid_.dev; }
   // stick buffer 0 or 1 sized for `columns` columns at precision `prec`
@@ -265,6 +268,7 @@ class DeviceOperator {
 
  private:
   void apply_chunk(int which, const double* x, long columns, double* hy, double* sy, Precision prec);
+  std::shared_ptr<void> davidson_cache_;
   Context& ctx_;
   std::size_t n_ = 0, np_ = 0, chunk_ = 256;
   long count_ = 0;
