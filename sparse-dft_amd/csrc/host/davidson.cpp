@@ -669,8 +669,7 @@ DavidsonOutput Solver::run(const C* initial) {
   const std::vector<long> order = davidson_order(op_);
   const bool warm = initial != nullptr && 2 * R < n;
   const long nseeds = davidson_seed_columns(n, R, warm);
-  if (warm)
-    check(hipMemcpyAsync(corr_.p, initial, 16 * std::size_t(n) * R, hipMemcpyHostToDevice, s_), "initial block");
+  if (warm) ctx_.upload_bulk(corr_.p, initial, 16 * std::size_t(n) * R);
   // the retained block holds cap_ columns; a local subspace of fewer than
   // 3 roots (band groups) takes a block of its own
   DevBuf<double> wide;
@@ -843,7 +842,7 @@ DavidsonOutput Solver::run(const C* initial) {
     out.eigenvectors.resize(std::size_t(n) * R);
     dst = out.eigenvectors.data();
   }
-  check(hipMemcpyAsync(dst, x_.p, 16 * std::size_t(n) * R, hipMemcpyDeviceToHost, s_), "vectors");
+  ctx_.download_bulk(dst, x_.p, 16 * std::size_t(n) * R);
   sync();
   stats_.iterations = iterations;
   stats_.rr_stedc = eig_ ? eig_->stedc_solves() - stedc_base_ : 0;

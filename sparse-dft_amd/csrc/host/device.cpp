@@ -195,6 +195,55 @@ PinnedStaging& Context::staging() {
   return *staging_;
 }
 
+namespace {
+constexpr std::size_t kBulkChunk = std::size_t(8) << 20;
+}
+
+void Context::bulk_buffers() {
+  if (bulk_[0]) return;
+  for (int i = 0; i < 2; ++i) {
+    check(hipHostMalloc(reinterpret_cast<void**>(&bulk_[i]), kBulkChunk, hipHostMallocDefault), "hipHostMalloc");
+    check(hipEventCreateWithFlags(&bulk_done_[i], hipEventDisableTiming), "hipEventCreate");
+  }
+}
+
+void Context::upload_bulk(void* device_dst, const void* host_src, std::size_t bytes) {
+  bulk_buffers();
+  const char* src = static_cast<const char*>(host_src);
+  char* dst = static_cast<char*>(device_dst);
+  bool pending[2] = {false, false};
+  for (std::size_t off = 0, k = 0; off < bytes; off += kBulkChunk, ++k) {
+    const int b = int(k & 1);
+    const std::size_t len = std::min(kBulkChunk, bytes - off);
+    if (pending[b]) check(hipEventSynchronize(bulk_done_[b]), "upload chunk");
+    std::memcpy(bulk_[b], src + off, len);
+    check(hipMemcpyAsync(dst + off, bulk_[b], len, hipMemcpyHostToDevice, stream_), "upload chunk");
+    check(hipEventRecord(bulk_done_[b], stream_), "hipEventRecord");
+    pending[b] = true;
+  }
+  sync();
+}
+
+void Context::download_bulk(void* host_dst, const void* device_src, std::size_t bytes) {
+  bulk_buffers();
+  const char* src = static_cast<const char*>(device_src);
+  char* dst = static_cast<char*>(host_dst);
+  const std::size_t chunks = (bytes + kBulkChunk - 1) / kBulkChunk;
+  auto issue = [&](std::size_t k) {
+    const std::size_t off = k * kBulkChunk, len = std::min(kBulkChunk, bytes - off);
+    check(hipMemcpyAsync(bulk_[k & 1], src + off, len, hipMemcpyDeviceToHost, stream_), "download chunk");
+    check(hipEventRecord(bulk_done_[k & 1], stream_), "hipEventRecord");
+  };
+  if (chunks) issue(0);
+  for (std::size_t k = 0; k < chunks; ++k) {
+    // chunk k + 1 lands in the other buffer while chunk k is copied out
+    if (k + 1 < chunks) issue(k + 1);
+    check(hipEventSynchronize(bulk_done_[k & 1]), "download chunk");
+    const std::size_t off = k * kBulkChunk, len = std::min(kBulkChunk, bytes - off);
+    std::memcpy(dst + off, bulk_[k & 1], len);
+  }
+}
+
 void Context::fork() {
   check(hipEventRecord(fork_event_, stream_), "fork record");
   check(hipStreamWaitEvent(aux_, fork_event_, 0), "fork wait");
@@ -211,6 +260,10 @@ Context::~Context() {
   for (hipEvent_t e : events_) (void)hipEventDestroy(e);
   if (fork_event_) (void)hipEventDestroy(fork_event_);
   if (join_event_) (void)hipEventDestroy(join_event_);
+  for (int i = 0; i < 2; ++i) {
+    if (bulk_[i]) (void)hipHostFree(bulk_[i]);
+    if (bulk_done_[i]) (void)hipEventDestroy(bulk_done_[i]);
+  }
   if (blas_) rocblas_destroy_handle(blas_);
   if (aux_) (void)hipStreamDestroy(aux_);
   if (stream_) (void)hipStreamDestroy(stream_);

@@ -113,6 +113,11 @@ class Context {
   hipStream_t aux_stream() const { return aux_; }
   // pinned staging of the main stream's control transfers (allocated once)
   PinnedStaging& staging();
+  // bulk transfers between pageable host memory and the device on the main
+  // stream through two pinned chunks: the DMA of one chunk overlaps the host
+  // copy of the other. Both return with the transfer complete.
+  void upload_bulk(void* device_dst, const void* host_src, std::size_t bytes);
+  void download_bulk(void* host_dst, const void* device_src, std::size_t bytes);
   void fork();
   void join();
   rocblas_handle blas() const { return blas_; }
@@ -168,6 +173,9 @@ class Context {
   hipStream_t aux_{};
   hipEvent_t fork_event_{}, join_event_{};
   std::unique_ptr<PinnedStaging> staging_;
+  char* bulk_[2] = {nullptr, nullptr};
+  hipEvent_t bulk_done_[2] = {};
+  void bulk_buffers();
   rocblas_handle blas_{};
   std::string name_;
   DevBuf<double> work_;
