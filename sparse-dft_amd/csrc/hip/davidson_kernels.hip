@@ -346,14 +346,29 @@ __global__ __launch_bounds__(TPB) void k_davidson_seeds(long n, long warm_cols, 
     col[2 * i + 1] = seed_uniform(base + 2 * i + 1);
   }
   __syncthreads();
+  // the sums run in index order; the loads of 32 entries are issued ahead of
+  // their (dependent) additions
+  auto ordered_norm = [n](const double* v) {
+    constexpr int kAhead = 32;
+    double nn = 0.;
+    long i = 0;
+    for (; i + kAhead <= n; i += kAhead) {
+      double re[kAhead], im[kAhead];
+#pragma unroll
+      for (int t = 0; t < kAhead; ++t) {
+        re[t] = v[2 * (i + t)];
+        im[t] = v[2 * (i + t) + 1];
+      }
+#pragma unroll
+      for (int t = 0; t < kAhead; ++t) nn += re[t] * re[t] + im[t] * im[t];
+    }
+    for (; i < n; ++i) nn += v[2 * i] * v[2 * i] + v[2 * i + 1] * v[2 * i + 1];
+    return sqrt(nn);
+  };
   if (threadIdx.x == 0) {
-    double nn = 0.;
-    for (long i = 0; i < n; ++i) nn += col[2 * i] * col[2 * i] + col[2 * i + 1] * col[2 * i + 1];
-    norms[0] = sqrt(nn);
+    norms[0] = ordered_norm(col);
   } else if (threadIdx.x == 64 && warm) {
-    double nn = 0.;
-    for (long i = 0; i < n; ++i) nn += src[2 * i] * src[2 * i] + src[2 * i + 1] * src[2 * i + 1];
-    norms[1] = sqrt(nn);
+    norms[1] = ordered_norm(src);
   }
   __syncthreads();
   const double nb = norms[0];
