@@ -6,6 +6,8 @@ by residual and orthogonality after the symmetric orthonormalisation the
 device applies, on the matrix families that break naive inverse iteration."""
 import ctypes as ct
 import pathlib
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -16,10 +18,17 @@ GAP = 1e-7  # kTridiagClusterGap
 
 
 @pytest.fixture(scope="module")
-def core():
-    if not LIB.exists():
-        pytest.fail(f"{LIB} not built (make -C sparse-dft_amd)")
-    lib = ct.CDLL(str(LIB))
+def core(tmp_path_factory):
+    path = LIB
+    if not path.exists():
+        # built by `make -C sparse-dft_amd`; compile a private copy otherwise
+        cxx = shutil.which("g++")
+        if cxx is None:
+            pytest.fail(f"{LIB} not built and no g++ to build it")
+        path = tmp_path_factory.mktemp("tridiag") / LIB.name
+        subprocess.run([cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-o", str(path),
+                        str(LIB.parent / "tridiag_core_check.cpp")], check=True)
+    lib = ct.CDLL(str(path))
     lib.tridiag_core_values.restype = ct.c_int
     lib.tridiag_core_vectors.restype = ct.c_int
     return lib
