@@ -361,16 +361,17 @@ __global__ void k_atom_blocks_f32(int np, int columns, int atoms, const int* __r
 
 // Tile configurations: 0 = 64 x 64 with prefetch (FP32), 1 = 64 x 64 without
 // (FP32 split-K slabs), 2 = 128 x 32 with prefetch (tall, narrow outputs:
-// rotations), 3 = 64 x 32 with prefetch (every other FP64 shape). Measured on
-// the Si64 FP64 shapes against the 64 x 64 / 128 x 32 plans: slabs V^H W
-// 800 x 160 x n 51.2 against 39.8 TFLOP/s, projections 512 x 160 x n 62.6 / 55.0,
-// back-projections n x 160 x 512 54.6 / 42.2, Gram 160 x 160 x n 38.9 / 30.0,
-// Rayleigh-Ritz 800^3 47.2 / 37.0; rotations n x 160 x 800 stay at 128 x 32
-// (65.5 against 62.6).
+// rotations), 3 = 32 x 32 with prefetch (every other FP64 shape: one 16 x 16
+// block per wave, the most workgroups). Measured on the Si64 FP64 shapes, in one
+// sweep, 32 x 32 / 64 x 32 / the earlier 64 x 64 or 128 x 32: slabs V^H W
+// 800 x 160 x n 57.6 / 46.3 / 39.8 TFLOP/s, back-projections n x 160 x 512
+// 63.4 / 57.9 / 42.2, Gram 160 x 160 x n 42.4 / 39.2 / 30.0, Rayleigh-Ritz 800^3
+// 55.6 / 47.9 / 37.0, projections 512 x 160 x n 65.8 / 66.8 / 55.0; rotations
+// n x 160 x 800 stay at 128 x 32 (67.6, 32 x 32 67.7).
 struct Tile {
   int bm, bn;
 };
-constexpr Tile kTiles[4] = {{64, 64}, {64, 64}, {128, 32}, {64, 32}};
+constexpr Tile kTiles[4] = {{64, 64}, {64, 64}, {128, 32}, {32, 32}};
 
 bool tall_shape(int M, int N) { return M >= 2048 && N <= 192 && long(M) >= 8L * N; }
 
@@ -428,7 +429,7 @@ int splits_for(long tiles, int K, long target) {
 long tile_count(int M, int N, int cfg) {
   return long((M + kTiles[cfg].bm - 1) / kTiles[cfg].bm) * ((N + kTiles[cfg].bn - 1) / kTiles[cfg].bn);
 }
-// FP64: 64 x 32 tiles and >= 2048 workgroups (tall shapes need no split);
+// FP64: 32 x 32 tiles and >= 2048 workgroups (tall shapes need no split);
 // rotations (tall, normal A) 128 x 32
 int fp64_cfg(int M, int N, int aop) { return tall_shape(M, N) && !aop ? 2 : 3; }
 int fp64_splits(int M, int N, int K, int aop) {
