@@ -359,19 +359,22 @@ __global__ void k_atom_blocks_f32(int np, int columns, int atoms, const int* __r
   }
 }
 
-// Tile configurations: 0 = 64 x 64 with prefetch (FP32), 1 = 64 x 64 without
-// (FP32 split-K slabs), 2 = 128 x 32 with prefetch (tall, narrow outputs:
-// rotations), 3 = 32 x 32 with prefetch (every other FP64 shape: one 16 x 16
-// block per wave, the most workgroups). Measured on the Si64 FP64 shapes, in one
-// sweep, 32 x 32 / 64 x 32 / the earlier 64 x 64 or 128 x 32: slabs V^H W
-// 800 x 160 x n 57.6 / 46.3 / 39.8 TFLOP/s, back-projections n x 160 x 512
-// 63.4 / 57.9 / 42.2, Gram 160 x 160 x n 42.4 / 39.2 / 30.0, Rayleigh-Ritz 800^3
-// 55.6 / 47.9 / 37.0, projections 512 x 160 x n 65.8 / 66.8 / 55.0; rotations
-// n x 160 x 800 stay at 128 x 32 (67.6, 32 x 32 67.7).
+// Tile configurations, both precisions, with register prefetch: 0 = 128 x 32
+// (tall, narrow outputs with normal A: rotations), 1 = 32 x 32 (every other
+// shape: one 16 x 16 block per wave, the most workgroups). Measured on the Si64
+// shapes, 32 x 32 / 64 x 32 / 64 x 64 (or 128 x 32), FP64 with the split counts
+// of fp64_splits: slabs V^H W 800 x 160 x n 57.6 / 46.3 / 39.8 TFLOP/s,
+// back-projections n x 160 x 512 63.4 / 57.9 / 42.2, Gram 160 x 160 x n
+// 42.4 / 39.2 / 30.0, Rayleigh-Ritz 800^3 55.6 / 47.9 / 37.0, projections
+// 512 x 160 x n 65.8 / 66.8 / 55.0, rotations n x 160 x 800 67.7 / - / 67.6;
+// FP32 (four real products, best split count of each): slabs 75.3 / 69.9 /
+// 68.3, back-projections 82.0 / 77.8 / 72.2, Gram 53.2 / 47.4 / 40.1,
+// Rayleigh-Ritz 68.1 / 59.4 / 59.5, projections 82.6 / 88.1 / 79.7, rotations
+// 86.0 / 89.2 / 77.8 (128 x 32: 90.5).
 struct Tile {
   int bm, bn;
 };
-constexpr Tile kTiles[4] = {{64, 64}, {64, 64}, {128, 32}, {32, 32}};
+constexpr Tile kTiles[2] = {{128, 32}, {32, 32}};
 
 bool tall_shape(int M, int N) { return M >= 2048 && N <= 192 && long(M) >= 8L * N; }
 
@@ -379,11 +382,11 @@ template <typename T, typename TA, typename TB, int CFG>
 hipError_t launch_cfg(int aop, int bop, dim3 grid, int M, int N, int K, const TA* A, long lda, const TB* B, long ldb,
                       double* C, long ldc, const double* D, long ldd, double alpha, double beta, int kps,
                       double* work, hipStream_t s) {
-  constexpr int TBM = kTiles[CFG].bm, TBN = kTiles[CFG].bn, WGM = CFG == 2 ? 4 : 2;
-  constexpr bool PF = CFG != 1;
-  // measured per plan on the Si64 shapes: 3M FP64 split-K slabs fastest at
-  // three waves, the prefetching plans at two (three spill there)
-  constexpr int WAVES = (sizeof(T) == 8 && SDFT_GEMM_GAUSS_FP64 && CFG != 1) ? 2 : 3;
+  constexpr int TBM = kTiles[CFG].bm, TBN = kTiles[CFG].bn, WGM = CFG == 0 ? 4 : 2;
+  constexpr bool PF = true;
+  // measured on the Si64 shapes: the 3M FP64 tiles at two waves per SIMD
+  // (three spill), the FP32 tiles at three
+  constexpr int WAVES = (sizeof(T) == 8 && SDFT_GEMM_GAUSS_FP64) ? 2 : 3;
 #define SDFT_LAUNCH(AA, BB)                                                                                      \
   hipLaunchKernelGGL((k_zgemm<T, TA, TB, AA, BB, TBM, TBN, WGM, PF, WAVES>), grid, dim3(256), 0, s, M, N, K, A, \
                      lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work)
@@ -404,12 +407,8 @@ hipError_t dispatch_ops(int cfg, int aop, int bop, dim3 grid, int M, int N, int 
     if (e != hipSuccess) return e;
   }
   hipError_t e;
-  if (cfg == 2) e = launch_cfg<T, TA, TB, 2>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work, s);
-  else if (cfg == 3 && sizeof(T) == 8)
-    e = launch_cfg<T, TA, TB, 3>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work, s);
-  else if (cfg == 1 && sizeof(T) == 8)
-    e = launch_cfg<T, TA, TB, 1>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work, s);
-  else e = launch_cfg<T, TA, TB, 0>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work, s);
+  if (cfg == 0) e = launch_cfg<T, TA, TB, 0>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work, s);
+  else e = launch_cfg<T, TA, TB, 1>(aop, bop, grid, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha, beta, kps, work, s);
   if (e != hipSuccess || !ev || !ev->end) return e;
   return hipEventRecord(ev->end, s);
 }
@@ -429,23 +428,22 @@ int splits_for(long tiles, int K, long target) {
 long tile_count(int M, int N, int cfg) {
   return long((M + kTiles[cfg].bm - 1) / kTiles[cfg].bm) * ((N + kTiles[cfg].bn - 1) / kTiles[cfg].bn);
 }
-// FP64: 32 x 32 tiles and >= 2048 workgroups (tall shapes need no split);
-// rotations (tall, normal A) 128 x 32
-int fp64_cfg(int M, int N, int aop) { return tall_shape(M, N) && !aop ? 2 : 3; }
-int fp64_splits(int M, int N, int K, int aop) {
-  const int cfg = fp64_cfg(M, N, aop);
-  return splits_for(tile_count(M, N, cfg), K, tall_shape(M, N) ? 512 : 2048);
+// 32 x 32 tiles split along K to >= 2048 (FP64) or >= 1024 (FP32)
+// workgroups, tall shapes to >= 512 (they need no split); rotations (tall,
+// normal A) 128 x 32
+int plan_cfg(int M, int N, int aop) { return tall_shape(M, N) && !aop ? 0 : 1; }
+int plan_splits(bool fp64, int M, int N, int K, int aop) {
+  const int cfg = plan_cfg(M, N, aop);
+  return splits_for(tile_count(M, N, cfg), K, tall_shape(M, N) ? 512 : fp64 ? 2048 : 1024);
 }
 Plan plan_for(bool fp64, int aop, int M, int N, int K, bool have_work) {
   Plan p{};
-  const bool tall = tall_shape(M, N);
-  const int cfg64 = fp64_cfg(M, N, aop);
-  p.splits = fp64 ? fp64_splits(M, N, K, aop) : splits_for(tile_count(M, N, tall ? 2 : 0), K, 512);
+  p.splits = plan_splits(fp64, M, N, K, aop);
   if (p.splits > 1 && !have_work) p.splits = 1;
   p.kps = (K + p.splits - 1) / p.splits;
   p.kps = ((p.kps + BK - 1) / BK) * BK;
   p.splits = (K + p.kps - 1) / p.kps;
-  p.cfg = fp64 ? cfg64 : (tall ? 2 : 0);
+  p.cfg = plan_cfg(M, N, aop);
   p.grid = dim3((M + kTiles[p.cfg].bm - 1) / kTiles[p.cfg].bm, (N + kTiles[p.cfg].bn - 1) / kTiles[p.cfg].bn,
                 p.splits);
   return p;
@@ -461,9 +459,9 @@ std::size_t zgemm_workspace_bytes(int M, int N, int K) {
 
 // the largest split count any plan of the shape uses (workspace sizing)
 int zgemm_splits(int M, int N, int K) {
-  const bool tall = tall_shape(M, N);
-  int splits = splits_for(tile_count(M, N, tall ? 2 : 0), K, 512);
-  for (int aop = 0; aop < 2; ++aop) splits = std::max(splits, fp64_splits(M, N, K, aop));
+  int splits = 1;
+  for (int aop = 0; aop < 2; ++aop)
+    for (int fp64 = 0; fp64 < 2; ++fp64) splits = std::max(splits, plan_splits(fp64, M, N, K, aop));
   return splits;
 }
 
