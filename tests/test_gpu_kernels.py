@@ -21,7 +21,10 @@ def naive_gemm(aop, bop, a, b):
     return A @ B
 
 
-@pytest.mark.parametrize("mnk", [(23, 19, 37), (127, 89, 367), (4097, 1, 1031), (64, 64, 16), (1, 1, 1)])
+# (100, 70, 2600): K split eight ways with a ragged last split; (4097, 1, 1031):
+# the tall plans (128 x 32 tiles for normal A), split along K
+@pytest.mark.parametrize("mnk", [(23, 19, 37), (127, 89, 367), (4097, 1, 1031), (64, 64, 16), (1, 1, 1),
+                                 (100, 70, 2600)])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_zgemm_all_ops_padded(hip_ctx, mnk, precision):
     m, n, k = mnk
