@@ -40,7 +40,21 @@ int sdft_hip_capabilities(char* buffer, size_t size);
 /* Context (HIP stream + rocBLAS/rocSOLVER handle + grow-only buffers); the
  * state a `hip` Accelerator owns (src/compute_cuda.cu:82-85 for CUDA). */
 int sdft_hip_context_create(int device, sdft_hip_context** out);
+/* Drops the creator's reference. Every operator built on the context holds a
+ * reference of its own, so the context's streams, handles and buffers are
+ * released when the last of them is destroyed, in whatever order the caller
+ * destroys them. */
 int sdft_hip_context_destroy(sdft_hip_context* ctx);
+/* Fault injection for tests of failure handling (no counterpart in the
+ * reference; every kernel still exits normally): the next `count` events of
+ * `kind` on this context misbehave on purpose. kind 0: the one-launch
+ * tridiagonalisation of a Rayleigh-Ritz solve finds its grid barrier expired
+ * (the solve is redone by rocSOLVER zheevd); kind 1: an operator application
+ * that produces H x writes a NaN into that output (the Davidson solve using it
+ * fails with a runtime error). count 0 clears the kind. */
+#define SDFT_HIP_FAULT_BARRIER_EXPIRY 0
+#define SDFT_HIP_FAULT_NONFINITE_APPLY 1
+int sdft_hip_inject_fault(sdft_hip_context* ctx, int kind, long count);
 /* Accelerator::device() (src/compute_backend.hpp:66) */
 int sdft_hip_device_name(sdft_hip_context* ctx, char* buffer, size_t size);
 int sdft_hip_synchronize(sdft_hip_context* ctx);
@@ -168,6 +182,14 @@ typedef struct {
    * (SDFT_BAND_GROUP_SOLVER=1 with SDFT_BAND_GROUP_SIZE, src/pw_core.cpp:564-567,
    * src/band_group_solver.cpp:57-211); 0: one block Davidson. */
   long band_group_size;
+  /* FP32 search only (no counterpart in the reference, which refuses FP32
+   * search): 0 continues with the reference's Davidson iterations in FP64
+   * after the hand-off; 1 replaces them by block Jacobi-Davidson refinement
+   * steps: FP32 preconditioned-CG correction solves against the fresh FP64
+   * residual, FP64 images of the corrections and an FP64 Rayleigh-Ritz of the
+   * Ritz block and the corrections. The certificate is the same. */
+  int refinement;
+  int inner_iterations; /* refinement: CG steps per correction solve, 0 = default */
 } sdft_hip_davidson_params;
 
 typedef struct {
@@ -184,6 +206,16 @@ typedef struct {
   long rr_cholesky;     /* Rayleigh-Ritz solves that took the Cholesky reduction */
   long rr_stedc;        /* Rayleigh-Ritz tridiagonal solves by divide and conquer (inverse iteration
                            failed its checks, or SDFT_HIP_TRIDIAG_EIGEN=stedc) */
+  long rr_barrier_fallbacks; /* eigensolves redone by rocSOLVER zheevd after the one-launch reduction's
+                                bounded grid barrier expired */
+  /* strict polish (src/strict_residual_polish.hpp:125-197) and root-manifold
+   * (:68-123) steps accepted / rejected on fresh FP64 images */
+  long polish_accepted, polish_rejected, manifold_accepted, manifold_rejected;
+  /* hand-offs from search arithmetic whose first fresh FP64 residual did not
+   * confirm the search residual (the role of the reference's counter
+   * davidson_coarse_original_residual_rejected) */
+  long search_rejected;
+  long refinement_steps; /* Jacobi-Davidson refinement steps after the hand-off */
 } sdft_hip_davidson_result;
 
 /* PlaneWaveOperator::project_into / projector_metric_into
@@ -255,7 +287,8 @@ int sdft_hip_hermitian_eigen(sdft_hip_context* ctx, size_t n, const double* a, s
                              double* vectors,
                              int* fused /* out, may be NULL: 1 = one-launch tridiagonalisation and inverse
                                            iteration, 2 = one-launch tridiagonalisation and divide and
-                                           conquer, 0 = zheevd */);
+                                           conquer, 0 = zheevd (also after the reduction's grid barrier
+                                           expired) */);
 
 /* Davidson seed block of the device solver (no counterpart as a function in
  * the reference; src/pw_core.cpp:1027-1112 builds it inline; exposed for
@@ -278,6 +311,19 @@ int sdft_hip_tridiagonal_eigen(sdft_hip_context* ctx, size_t n, const double* d,
 
 int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* params, const double* initial,
                       sdft_hip_davidson_result* result);
+
+/* One strict certificate-path step on a given block (the reference's
+ * strict_residual_polish_step and strict_root_manifold_step primitives,
+ * src/strict_residual_polish.hpp:68-197, as the device Davidson runs them;
+ * exposed for tests): the block x (host, n x roots, in/out) gets a fresh FP64
+ * certificate (S normalisation, H/S applications, energies, residuals), then
+ * kind 0 tries the tangent polish along `direction` (host, n x roots), kind 1
+ * the re-Ritz inside the block. A step is kept only when the fresh residual
+ * improves and the S constraints hold; *accepted says whether it was. On
+ * return x, energies and residual_norms describe the kept block. A nonfinite
+ * direction is an invalid argument. */
+int sdft_hip_strict_step(sdft_hip_operator* op, int kind, size_t roots, double tolerance, double* x,
+                         const double* direction, double* energies, double* residual_norms, int* accepted);
 
 /* Compensated per-atom onsite products W = [D_a P_a | Q_a P_a] on the device
  * (src/paw_compensated_product.hpp:110-135 order), host pointers; for tests. */

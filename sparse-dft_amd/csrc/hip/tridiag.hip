@@ -388,7 +388,7 @@ size_t hetrd_workspace_bytes(int n) {
 }
 
 hipError_t launch_hetrd(const double* a, long lda, int n, double* refl, long ldr, double* d, double* e, double* tau,
-                        void* workspace, hipStream_t s) {
+                        void* workspace, hipStream_t s, bool force_expiry) {
   int rpw = 0, groups = 0;
   if (!hetrd_plan(n, &rpw, &groups)) return hipErrorInvalidValue;
   if (n == 1) {
@@ -401,6 +401,9 @@ hipError_t launch_hetrd(const double* a, long lda, int n, double* refl, long ldr
   unsigned* flags = reinterpret_cast<unsigned*>(abort_flag) + kFlagStride;
   hipError_t err = hipMemsetAsync(abort_flag, 0, size_t(kFlagStride) * sizeof(unsigned) * (1 + size_t(groups)), s);
   if (err != hipSuccess) return err;
+  // host-forced expiry (fault injection): the abort flag is raised before the
+  // launch, so every workgroup leaves at its first wait and the kernel ends
+  if (force_expiry && (err = hipMemsetAsync(abort_flag, 1, 1, s)) != hipSuccess) return err;
   const Z* az = reinterpret_cast<const Z*>(a);
   Z* rz = reinterpret_cast<Z*>(refl);
   Z* tz = reinterpret_cast<Z*>(tau);

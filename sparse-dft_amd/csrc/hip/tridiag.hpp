@@ -19,9 +19,10 @@ size_t hetrd_workspace_bytes(int n);
 // (column k, rows k+2..n-1; other entries untouched). refl must not alias a.
 // `workspace` holds hetrd_workspace_bytes(n); after the launch completes the
 // int at hetrd_abort_flag(workspace, n) is nonzero if the grid barrier expired
-// (the outputs are then invalid).
+// (the outputs are then invalid). force_expiry raises that flag before the
+// launch (fault injection: the workgroups leave at their first wait).
 hipError_t launch_hetrd(const double* a, long lda, int n, double* refl, long ldr, double* d, double* e, double* tau,
-                        void* workspace, hipStream_t s);
+                        void* workspace, hipStream_t s, bool force_expiry = false);
 const int* hetrd_abort_flag(const void* workspace, int n);
 
 }  // namespace sdft_hip

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -20,14 +21,39 @@
 #include "pw_basis.hpp"
 #include "snapshot.hpp"
 
+// A context is reference counted: its creator holds one reference and every
+// operator built on it holds another, so sdft_hip_context_destroy of a context
+// whose operators are still alive only drops the creator's reference, and the
+// HIP streams, handles and buffers go when the last operator is destroyed.
+// Operators hold a reference to their context's state (device.hpp), so no
+// destruction order of the caller can leave an operator on a freed context.
 struct sdft_hip_context {
+  std::atomic<long> refs{1};
   sdft_hip::Context impl;
   explicit sdft_hip_context(int device) : impl(device) {}
 };
+
+namespace {
+void retain(sdft_hip_context* c) { c->refs.fetch_add(1, std::memory_order_relaxed); }
+void release(sdft_hip_context* c) {
+  if (c->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete c;
+}
+// the owning reference of an operator: declared before the operator state, so
+// the state is destroyed first and the context released after it
+struct ContextRef {
+  sdft_hip_context* ctx;
+  explicit ContextRef(sdft_hip_context* c) : ctx(c) { retain(c); }
+  ~ContextRef() { release(ctx); }
+  ContextRef(const ContextRef&) = delete;
+  ContextRef& operator=(const ContextRef&) = delete;
+};
+}  // namespace
+
 struct sdft_hip_operator {
+  ContextRef owner;
   sdft_hip::DeviceOperator impl;
-  sdft_hip_operator(sdft_hip::Context& c, const sdft_hip::OperatorDesc& d) : impl(c, d) {}
-  sdft_hip_operator(sdft_hip::Context& c, const sdft_hip::SnapshotArrays& s) : impl(c, s) {}
+  sdft_hip_operator(sdft_hip_context* c, const sdft_hip::OperatorDesc& d) : owner(c), impl(c->impl, d) {}
+  sdft_hip_operator(sdft_hip_context* c, const sdft_hip::SnapshotArrays& s) : owner(c), impl(c->impl, s) {}
 };
 
 namespace {
@@ -104,7 +130,16 @@ int sdft_hip_context_create(int device, sdft_hip_context** out) {
 }
 
 int sdft_hip_context_destroy(sdft_hip_context* ctx) {
-  return guarded([&] { delete ctx; });
+  return guarded([&] {
+    if (ctx) release(ctx);
+  });
+}
+
+int sdft_hip_inject_fault(sdft_hip_context* ctx, int kind, long count) {
+  return guarded([&] {
+    if (!ctx) throw std::invalid_argument("HIP: null context");
+    ctx->impl.inject_fault(kind, count);
+  });
 }
 
 int sdft_hip_device_name(sdft_hip_context* ctx, char* buffer, size_t size) {
@@ -388,7 +423,7 @@ int sdft_hip_operator_create(sdft_hip_context* ctx, const sdft_hip_operator_desc
     d.q = desc->q;
     d.regularization = desc->regularization;
     d.shape = {desc->shape[0], desc->shape[1], desc->shape[2]};
-    *out = new sdft_hip_operator(ctx->impl, d);
+    *out = new sdft_hip_operator(ctx, d);
   });
 }
 
@@ -517,6 +552,10 @@ int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* par
     p.verbose = params->verbose;
     p.timing = params->timing;
     p.band_group_size = params->band_group_size;
+    p.refinement = params->refinement;
+    p.inner_iterations = params->inner_iterations;
+    if (p.refinement != 0 && p.refinement != 1) throw std::invalid_argument("HIP: refinement must be 0 or 1");
+    if (p.inner_iterations < 0) throw std::invalid_argument("HIP: inner iterations must be >= 0");
     p.vectors_out = reinterpret_cast<std::complex<double>*>(result->eigenvectors);
     if (p.search_fp32 != 0 && p.search_fp32 != 1) throw std::invalid_argument("HIP: search precision must be 0 or 1");
     const auto out = sdft_hip::davidson(op->impl, p, reinterpret_cast<const std::complex<double>*>(initial));
@@ -536,6 +575,28 @@ int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* par
     result->cholesky_blocks = out.stats.cholesky_blocks;
     result->rr_cholesky = out.stats.rr_cholesky;
     result->rr_stedc = out.stats.rr_stedc;
+    result->rr_barrier_fallbacks = out.stats.rr_barrier_fallbacks;
+    result->polish_accepted = out.stats.polish_accepted;
+    result->polish_rejected = out.stats.polish_rejected;
+    result->manifold_accepted = out.stats.manifold_accepted;
+    result->manifold_rejected = out.stats.manifold_rejected;
+    result->search_rejected = out.stats.search_rejected;
+    result->refinement_steps = out.stats.refinement_steps;
+  });
+}
+
+int sdft_hip_strict_step(sdft_hip_operator* op, int kind, size_t roots, double tolerance, double* x,
+                         const double* direction, double* energies, double* residual_norms, int* accepted) {
+  return guarded([&] {
+    if (!op || !x || !energies || !residual_norms || !accepted)
+      throw std::invalid_argument("invalid strict residual polish dimensions");
+    using Cx = std::complex<double>;
+    const auto r = sdft_hip::strict_step(op->impl, kind, long(roots), tolerance, reinterpret_cast<const Cx*>(x),
+                                         reinterpret_cast<const Cx*>(direction));
+    std::memcpy(x, r.x.data(), r.x.size() * sizeof(Cx));
+    std::copy(r.energies.begin(), r.energies.end(), energies);
+    std::copy(r.residual_norms.begin(), r.residual_norms.end(), residual_norms);
+    *accepted = r.accepted ? 1 : 0;
   });
 }
 
@@ -647,7 +708,7 @@ int sdft_hip_snapshot_load(sdft_hip_context* ctx, const char* path, double* init
   return guarded([&] {
     if (!ctx || !path || !out) throw std::invalid_argument("HIP: invalid snapshot arguments");
     const sdft_hip::SnapshotArrays s = sdft_hip::read_snapshot(path);
-    auto op = std::make_unique<sdft_hip_operator>(ctx->impl, s);
+    auto op = std::make_unique<sdft_hip_operator>(ctx, s);
     if (initial && !s.initial.empty()) std::memcpy(initial, s.initial.data(), s.initial.size() * sizeof(s.initial[0]));
     *out = op.release();
   });
@@ -680,7 +741,8 @@ int sdft_hip_hermitian_eigen(sdft_hip_context* ctx, size_t n, const double* a, s
     std::copy(w.begin(), w.end(), values);
     if (k) sdft_hip::check(hipMemcpyAsync(vectors, vec.p, 16 * n * k, hipMemcpyDeviceToHost, s), "download");
     cx.sync();
-    if (fused) *fused = eig.launches_fused() > 0 ? (eig.stedc_solves() > 0 ? 2 : 1) : 0;
+    if (fused)
+      *fused = eig.launches_fused() > 0 && eig.barrier_fallbacks() == 0 ? (eig.stedc_solves() > 0 ? 2 : 1) : 0;
   });
 }
 

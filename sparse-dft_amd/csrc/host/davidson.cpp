@@ -92,6 +92,7 @@ class Solver {
     }
     eig_ = ws_->eig.get();
     stedc_base_ = eig_->stedc_solves();
+    barrier_base_ = eig_->barrier_fallbacks();
     if (p.excluded_count < 0 || (p.excluded_count && !p.excluded))
       throw std::invalid_argument("invalid completed-band dimensions");
     excl_ = p.excluded_count;
@@ -105,6 +106,10 @@ class Solver {
   Solver(const Solver&) = delete;
   Solver& operator=(const Solver&) = delete;
   DavidsonOutput run(const C* initial);
+  // One strict step on a given block, from a fresh FP64 certificate of it (the
+  // reference's strict_residual_polish_step / strict_root_manifold_step
+  // primitives, src/strict_residual_polish.hpp:68-197, without locked bands).
+  StrictStepOutput strict_primitive(int kind, const C* x, const C* direction);
 
  private:
   Precision search_prec() const { return exact_ ? Precision::FP64 : Precision::FP32; }
@@ -217,7 +222,16 @@ class Solver {
     double* e2 = m3_.p + 2 * m * m;
     double* mm = qbuf_.p;
     double* t = m2_.p;
-    const double defect = identity_defect(b, m, m, e);
+    // max |B - I| and, as a finiteness check of the projected Hamiltonian,
+    // max |A - I| (a NaN dominates both maxima), read in one transfer
+    auto* bits = reinterpret_cast<unsigned long long*>(small_.p);
+    check(hipMemsetAsync(bits, 0, 2 * sizeof(unsigned long long), s_), "memset");
+    check(launch_identity_defect(b, m, m, e, m, bits, s_), "identity defect");
+    check(launch_identity_defect(a, m, m, nullptr, m, bits + 1, s_), "finite check");
+    const std::vector<double> defects = to_host(small_.p, 2);
+    const double defect = defects[0];
+    if (!std::isfinite(defect) || !std::isfinite(defects[1]))
+      throw std::runtime_error("HIP: nonfinite Davidson projected matrices (an operator application was not finite)");
     if (!(defect <= 1e-4)) {
       ++stats_.rr_cholesky;
       if (p_.timing) stats_.seconds_ritz += now() - t0;
@@ -311,11 +325,13 @@ class Solver {
   DevBuf<double>&small_, &w1_, &w2_, &w3_, &w4_, &m1_, &m2_, &m3_, &rr_, &rrb_, &qbuf_, &corr_, &retained_;
   HermitianEigensolver* eig_ = nullptr;
   long stedc_base_ = 0;  // divide-and-conquer solves of the shared eigensolver before this solve
+  long barrier_base_ = 0;  // its barrier-expiry fallbacks before this solve
   PinnedStaging& stage_ = ctx_.staging();
   long excl_ = 0;
   const double* excluded_ = nullptr;
   DevBuf<double> excluded_s_, excl_coef_;
   long size_ = 0, computed_h_ = 0, projected_size_ = 0;
+  double search_residual_ = -1;  // max residual of the search arithmetic at the hand-off, until confirmed
   std::vector<double> values_, res_norms_;
   DavidsonStats stats_;
 };
@@ -659,6 +675,32 @@ void Solver::preserved_subspace(long keep, double* retained) {
   gemm(true, 0, 0, n_, keep - R_, m, 1.0, basis_.p, n_, q + 2 * R_ * m, m, retained + 2 * R_ * n_, n_);
 }
 
+StrictStepOutput Solver::strict_primitive(int kind, const C* x, const C* direction) {
+  const long n = n_, R = R_;
+  ctx_.upload_bulk(x_.p, x, 16 * std::size_t(n) * R);
+  values_.assign(R, 0.);
+  res_norms_.assign(R, 0.);
+  certify();
+  StrictStepOutput out;
+  if (kind == 0) {
+    ctx_.upload_bulk(corr_.p, direction, 16 * std::size_t(n) * R);
+    // the flag is an int in the first word of a zeroed double
+    int* flag = reinterpret_cast<int*>(small_.p);
+    check(hipMemsetAsync(small_.p, 0, sizeof(double), s_), "memset");
+    check(launch_finite_check(corr_.p, n, n, R, flag, s_), "finite");
+    if (to_host(small_.p, 1)[0] != 0.0) throw std::invalid_argument("nonfinite strict residual polish state");
+    out.accepted = polish_step(corr_.p);
+  } else {
+    out.accepted = manifold_step();
+  }
+  refresh_norms();
+  out.x.resize(std::size_t(n) * R);
+  ctx_.download_bulk(out.x.data(), x_.p, 16 * std::size_t(n) * R);
+  out.energies = values_;
+  out.residual_norms = res_norms_;
+  return out;
+}
+
 DavidsonOutput Solver::run(const C* initial) {
   const double t_start = now();
   const long n = n_, R = R_;
@@ -710,6 +752,12 @@ DavidsonOutput Solver::run(const C* initial) {
     refresh_norms();
     iterations = it + 1;
     stats_.max_residual_history.push_back(max_res());
+    if (exact_ && search_residual_ >= 0) {
+      // first Rayleigh-Ritz on FP64 images after the hand-off: the search
+      // arithmetic's residual must be confirmed by the original operator
+      if (max_res() > std::max(8 * tol_, 10 * search_residual_)) ++stats_.search_rejected;
+      search_residual_ = -1;
+    }
     if (p_.verbose)
       std::fprintf(stderr, "davidson it=%ld size=%ld exact=%d max_residual=%.3e\n", it + 1, size_, int(exact_), max_res());
     if (!exact_) {
@@ -721,6 +769,7 @@ DavidsonOutput Solver::run(const C* initial) {
       const bool stalled = since_halving >= p_.stagnation_window;
       if (max_res() < p_.search_handoff || it + 1 == p_.max_iterations || stalled) {
         stats_.handoff_iteration = it + 1;
+        search_residual_ = max_res();
         const long keep = std::min(size_, long(std::max(2, p_.handoff_restart)) * R);
         if (p_.handoff_restart && size_ > keep) {
           // keep the lowest Ritz vectors (FP64 rotation), then the exact block
@@ -762,7 +811,9 @@ DavidsonOutput Solver::run(const C* initial) {
         ++manifold_attempts;
         ++it;
         iterations = it + 1;
-        polished |= manifold_step();
+        const bool accepted = manifold_step();
+        ++(accepted ? stats_.manifold_accepted : stats_.manifold_rejected);
+        polished |= accepted;
         refresh_norms();
       }
       for (; !converged_ && polish_attempts < 16 && it + 1 < p_.max_iterations; ++polish_attempts) {
@@ -776,6 +827,7 @@ DavidsonOutput Solver::run(const C* initial) {
         ++it;
         iterations = it + 1;
         const bool improved = polish_step(corr.p);
+        ++(improved ? stats_.polish_accepted : stats_.polish_rejected);
         polished |= improved;
         if (!improved) { ++polish_attempts; break; }
         refresh_norms();
@@ -846,6 +898,7 @@ DavidsonOutput Solver::run(const C* initial) {
   sync();
   stats_.iterations = iterations;
   stats_.rr_stedc = eig_ ? eig_->stedc_solves() - stedc_base_ : 0;
+  stats_.rr_barrier_fallbacks = eig_ ? eig_->barrier_fallbacks() - barrier_base_ : 0;
   stats_.converged = converged_;
   stats_.seconds_total = now() - t_start;
   out.stats = stats_;
@@ -885,6 +938,19 @@ long davidson_seed_block(DeviceOperator& op, long roots, double tol, const doubl
   // guard_dev is released on return
   check(hipStreamSynchronize(s), "seeds");
   return davidson_seed_columns(n, roots, warm);
+}
+
+StrictStepOutput strict_step(DeviceOperator& op, int kind, long roots, double tolerance, const std::complex<double>* x,
+                             const std::complex<double>* direction) {
+  if ((kind != 0 && kind != 1) || !x || (kind == 0 && !direction) || roots <= 0 || long(op.n()) < roots ||
+      !std::isfinite(tolerance) || tolerance <= 0)
+    throw std::invalid_argument("invalid strict residual polish dimensions");
+  DavidsonParams p;
+  p.roots = roots;
+  p.tolerance = tolerance;
+  p.max_iterations = 1;
+  Solver s(op, p);
+  return s.strict_primitive(kind, x, direction);
 }
 
 DavidsonOutput davidson(DeviceOperator& op, const DavidsonParams& params, const std::complex<double>* initial) {

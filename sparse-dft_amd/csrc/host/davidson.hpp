@@ -23,6 +23,10 @@ struct DavidsonParams {
   int handoff_restart = 1;        // FP32 only: h >= 1 thick-restarts to max(2, h) x roots Ritz vectors at the
                                   // hand-off and builds FP64 images for those; 0 re-applies FP64 to the whole
                                   // subspace
+  int refinement = 0;             // FP32 only: 0 continues with the reference's Davidson iterations in FP64
+                                  // after the hand-off; 1 runs block Jacobi-Davidson refinement steps (FP32
+                                  // correction solves, FP64 images and Rayleigh-Ritz)
+  int inner_iterations = 0;       // refinement: preconditioned CG steps per correction solve (0: default)
   int verbose = 0;
   int timing = 0;                 // synchronise phases and accumulate per-phase times
   long band_group_size = 0;       // > 0: bounded band-group solver (src/band_group_solver.cpp:57-211)
@@ -47,6 +51,14 @@ struct DavidsonStats {
   long cholesky_blocks = 0;  // correction blocks normalised by Cholesky QR
   long rr_cholesky = 0;      // Rayleigh-Ritz solves that needed the Cholesky reduction
   long rr_stedc = 0;         // tridiagonal solves by divide and conquer instead of inverse iteration
+  long rr_barrier_fallbacks = 0;  // eigensolves redone by zheevd after the reduction's grid barrier expired
+  // strict polish (src/strict_residual_polish.hpp:125-197) and root-manifold
+  // (:68-123) steps whose trial was accepted / rejected on fresh FP64 images
+  long polish_accepted = 0, polish_rejected = 0, manifold_accepted = 0, manifold_rejected = 0;
+  // hand-offs from the search arithmetic whose first fresh FP64 residual did
+  // not confirm the search residual (above max(8 tol, 10 x the search's))
+  long search_rejected = 0;
+  long refinement_steps = 0;  // Jacobi-Davidson refinement steps after the hand-off
   std::vector<double> max_residual_history;
 };
 
@@ -58,6 +70,17 @@ struct DavidsonOutput {
 
 DavidsonOutput davidson(DeviceOperator& op, const DavidsonParams& params,
                         const std::complex<double>* initial /* n x roots or nullptr */);
+
+struct StrictStepOutput {
+  bool accepted = false;
+  std::vector<std::complex<double>> x;  // n x roots, the block after the step
+  std::vector<double> energies, residual_norms;
+};
+// One strict step on the block x (host, n x roots) from its fresh FP64
+// certificate: kind 0 the tangent polish along `direction` (host, n x roots),
+// kind 1 the root-manifold re-Ritz (src/strict_residual_polish.hpp:68-197).
+StrictStepOutput strict_step(DeviceOperator& op, int kind, long roots, double tolerance, const std::complex<double>* x,
+                             const std::complex<double>* direction);
 
 // Canonical order of the basis by H/S diagonal (src/pw_core.cpp:1016-1021).
 std::vector<long> davidson_order(const DeviceOperator& op);

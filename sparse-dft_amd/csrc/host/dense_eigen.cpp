@@ -55,37 +55,70 @@ std::vector<double> HermitianEigensolver::solve(const double* a, long m, long k,
   double* tau = e + m;  // m complex
   int rpw = 0, groups = 0;
   const bool fused = hetrd_plan(int(m), &rpw, &groups);
-  if (fused) {
-    check(launch_hetrd(a, m, int(m), refl_.p, m, d, e, tau, work_.p, s), "tridiagonalisation");
-    ++fused_;
-    if (!use_stedc() && m <= kTridiagEigMaxOrder) {
-      std::vector<double> w;
-      if (inverse_iteration(m, k, count, vecs, w)) return w;
-      ++fallbacks_;
+  if (!fused) return solve_zheevd(a, m, k, count, vecs);
+  const bool force_expiry = ctx_.take_fault(Context::kFaultBarrierExpiry);
+  check(launch_hetrd(a, m, int(m), refl_.p, m, d, e, tau, work_.p, s, force_expiry), "tridiagonalisation");
+  ++fused_;
+  if (!use_stedc() && m <= kTridiagEigMaxOrder) {
+    std::vector<double> w;
+    bool aborted = false;
+    if (inverse_iteration(m, k, count, vecs, w, &aborted)) return w;
+    if (aborted) {
+      // `a` is unchanged: the whole solve is redone by divide and conquer
+      ++barrier_fallbacks_;
+      return solve_zheevd(a, m, k, count, vecs);
     }
-    check(rocsolver_zstedc(ctx_.blas(), rocblas_evect_tridiagonal, rocblas_int(m), d, e, Zm, rocblas_int(m), info),
-          "rocsolver_zstedc");
-    ++stedc_;
-    if (k > 0 && m > 1)
-      check(rocsolver_zunmtr(ctx_.blas(), rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                             rocblas_int(m), rocblas_int(k), reinterpret_cast<rocblas_double_complex*>(refl_.p),
-                             rocblas_int(m), reinterpret_cast<rocblas_double_complex*>(tau), Zm, rocblas_int(m)),
-            "rocsolver_zunmtr");
-  } else {
-    check(hipMemcpyAsync(z_.p, a, 16 * std::size_t(m) * m, hipMemcpyDeviceToDevice, s), "copy");
-    check(rocsolver_zheevd(ctx_.blas(), rocblas_evect_original, rocblas_fill_lower, rocblas_int(m), Zm,
-                           rocblas_int(m), d, e, info),
-          "rocsolver_zheevd");
+    ++fallbacks_;
   }
+  check(rocsolver_zstedc(ctx_.blas(), rocblas_evect_tridiagonal, rocblas_int(m), d, e, Zm, rocblas_int(m), info),
+        "rocsolver_zstedc");
+  ++stedc_;
+  if (k > 0 && m > 1)
+    check(rocsolver_zunmtr(ctx_.blas(), rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                           rocblas_int(m), rocblas_int(k), reinterpret_cast<rocblas_double_complex*>(refl_.p),
+                           rocblas_int(m), reinterpret_cast<rocblas_double_complex*>(tau), Zm, rocblas_int(m)),
+          "rocsolver_zunmtr");
   if (k > 0) check(hipMemcpyAsync(vecs, z_.p, 16 * std::size_t(m) * k, hipMemcpyDeviceToDevice, s), "vectors");
   std::vector<double> h(8 + m);
   check(hipMemcpyAsync(h.data(), small_.p, 8 * std::size_t(8 + m), hipMemcpyDeviceToHost, s), "eigenvalues");
+  if (barrier_expired(m)) {
+    ++barrier_fallbacks_;
+    return solve_zheevd(a, m, k, count, vecs);
+  }
+  int status = 0;
+  std::memcpy(&status, h.data(), sizeof(int));
+  if (status) throw std::runtime_error("HIP: Hermitian eigensolve failed");
+  return std::vector<double>(h.begin() + 8, h.begin() + 8 + count);
+}
+
+// Synchronises the stream; true when the reduction's bounded grid barrier
+// expired (its outputs are then invalid).
+bool HermitianEigensolver::barrier_expired(long m) {
   int aborted = 0;
-  if (fused && m > 1)
-    check(hipMemcpyAsync(&aborted, hetrd_abort_flag(work_.p, int(m)), sizeof(int), hipMemcpyDeviceToHost, s),
+  if (m > 1)
+    check(hipMemcpyAsync(&aborted, hetrd_abort_flag(work_.p, int(m)), sizeof(int), hipMemcpyDeviceToHost,
+                         ctx_.stream()),
           "abort flag");
   ctx_.sync();
-  if (aborted) throw std::runtime_error("HIP: tridiagonalisation grid barrier expired");
+  return aborted != 0;
+}
+
+// rocSOLVER zheevd on a copy of `a` (orders above the one-launch reduction's
+// limit, and the recovery after an expired grid barrier).
+std::vector<double> HermitianEigensolver::solve_zheevd(const double* a, long m, long k, long count, double* vecs) {
+  hipStream_t s = ctx_.stream();
+  auto* Zm = reinterpret_cast<rocblas_double_complex*>(z_.p);
+  int* info = reinterpret_cast<int*>(small_.p);
+  double* d = small_.p + 8;
+  double* e = d + m;
+  check(hipMemcpyAsync(z_.p, a, 16 * std::size_t(m) * m, hipMemcpyDeviceToDevice, s), "copy");
+  check(rocsolver_zheevd(ctx_.blas(), rocblas_evect_original, rocblas_fill_lower, rocblas_int(m), Zm, rocblas_int(m),
+                         d, e, info),
+        "rocsolver_zheevd");
+  if (k > 0) check(hipMemcpyAsync(vecs, z_.p, 16 * std::size_t(m) * k, hipMemcpyDeviceToDevice, s), "vectors");
+  std::vector<double> h(8 + m);
+  check(hipMemcpyAsync(h.data(), small_.p, 8 * std::size_t(8 + m), hipMemcpyDeviceToHost, s), "eigenvalues");
+  ctx_.sync();
   int status = 0;
   std::memcpy(&status, h.data(), sizeof(int));
   if (status) throw std::runtime_error("HIP: Hermitian eigensolve failed");
@@ -100,8 +133,8 @@ std::vector<double> HermitianEigensolver::solve(const double* a, long m, long k,
 // eigenvectors by their overlap c, which inverse iteration bounds by
 // O(eps |T| / gap), changes their residuals by c gap = O(eps |T|). Returns false
 // (outputs invalid) when an iteration did not stay finite or |E| > kMaxDefect.
-bool HermitianEigensolver::inverse_iteration(long m, long k, long count, double* vecs,
-                                             std::vector<double>& values) {
+bool HermitianEigensolver::inverse_iteration(long m, long k, long count, double* vecs, std::vector<double>& values,
+                                             bool* aborted_out) {
   hipStream_t s = ctx_.stream();
   int* status = reinterpret_cast<int*>(small_.p);
   auto* defect_bits = reinterpret_cast<unsigned long long*>(small_.p + 2);
@@ -138,12 +171,10 @@ bool HermitianEigensolver::inverse_iteration(long m, long k, long count, double*
   check(hipMemcpyAsync(values.data(), w, 8 * std::size_t(count), hipMemcpyDeviceToHost, s), "eigenvalues");
   double head[3] = {0, 0, 0};
   check(hipMemcpyAsync(head, small_.p, sizeof(head), hipMemcpyDeviceToHost, s), "status");
-  int aborted = 0;
-  if (m > 1)
-    check(hipMemcpyAsync(&aborted, hetrd_abort_flag(work_.p, int(m)), sizeof(int), hipMemcpyDeviceToHost, s),
-          "abort flag");
-  ctx_.sync();
-  if (aborted) throw std::runtime_error("HIP: tridiagonalisation grid barrier expired");
+  if (barrier_expired(m)) {
+    *aborted_out = true;
+    return false;
+  }
   int st = 0;
   std::memcpy(&st, head, sizeof(int));
   unsigned long long bits = 0;

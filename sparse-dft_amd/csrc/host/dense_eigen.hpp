@@ -22,14 +22,20 @@ class HermitianEigensolver {
   // divide and conquer; tridiagonal solves by divide and conquer in total
   long fallbacks() const { return fallbacks_; }
   long stedc_solves() const { return stedc_; }
+  // solves whose one-launch reduction found its bounded grid barrier expired
+  // (CUs held by other work past the wait bound) and that were redone by
+  // rocSOLVER zheevd on the unchanged input
+  long barrier_fallbacks() const { return barrier_fallbacks_; }
 
  private:
   Context& ctx_;
   long max_order_;
-  bool inverse_iteration(long m, long k, long count, double* vecs, std::vector<double>& values);
+  bool inverse_iteration(long m, long k, long count, double* vecs, std::vector<double>& values, bool* aborted);
+  std::vector<double> solve_zheevd(const double* a, long m, long k, long count, double* vecs);
+  bool barrier_expired(long m);
   DevBuf<double> refl_, z_, small_, orth_;
   DevBuf<char> work_;
-  long fused_ = 0, fallbacks_ = 0, stedc_ = 0;
+  long fused_ = 0, fallbacks_ = 0, stedc_ = 0, barrier_fallbacks_ = 0;
 };
 
 }  // namespace sdft_hip

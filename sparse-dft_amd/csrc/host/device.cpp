@@ -269,6 +269,18 @@ Context::~Context() {
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+void Context::inject_fault(int kind, long count) {
+  if (kind < 0 || kind >= kFaultKinds || count < 0) throw std::invalid_argument("HIP: invalid fault injection");
+  faults_[kind].store(count, std::memory_order_relaxed);
+}
+
+bool Context::take_fault(int kind) {
+  long v = faults_[kind].load(std::memory_order_relaxed);
+  while (v > 0)
+    if (faults_[kind].compare_exchange_weak(v, v - 1, std::memory_order_relaxed)) return true;
+  return false;
+}
+
 void Context::record(const std::string& name, double seconds, std::size_t calls) {
   auto& c = counters[name];
   c.seconds += seconds;
@@ -671,6 +683,10 @@ void DeviceOperator::apply(int which, const double* x, long columns, double* hy,
     const long c = std::min<long>(long(chunk_), columns - c0);
     const long off = 2 * c0 * long(n_);
     apply_chunk(which, x + off, c, hy ? hy + off : nullptr, sy ? sy + off : nullptr, prec);
+  }
+  if (which != 1 && ctx_.take_fault(Context::kFaultNonfiniteApply)) {
+    // injected fault: all-ones bits are a NaN double
+    check(hipMemsetAsync(hy, 0xFF, sizeof(double), ctx_.stream()), "fault injection");
   }
 }
 

@@ -6,6 +6,7 @@
 #include <rocblas/rocblas.h>
 
 #include <array>
+#include <atomic>
 #include <complex>
 #include <cstdint>
 #include <map>
@@ -131,6 +132,17 @@ class Context {
   void record(const std::string& name, double seconds, std::size_t calls = 1);
   std::map<std::string, PerfCounter> counters;
 
+  // Fault injection (tests of failure handling; every kernel still exits
+  // normally): the next `count` events of a kind misbehave on purpose.
+  //   kFaultBarrierExpiry: the one-launch tridiagonalisation finds its grid
+  //     barrier expired (the abort flag is raised from the host before launch);
+  //   kFaultNonfiniteApply: an operator application that produces H x writes
+  //     a NaN into the first entry of that output.
+  enum FaultKind { kFaultBarrierExpiry = 0, kFaultNonfiniteApply = 1, kFaultKinds = 2 };
+  void inject_fault(int kind, long count);
+  // consumes one pending fault of `kind`; true when one was pending
+  bool take_fault(int kind);
+
   // Kernel clock: event pairs on this context's stream around every MFMA GEMM
   // kernel and every local-Hamiltonian pipeline launch, summed per kind.
   enum ClockKind { kGemmFP64 = 0, kGemmFP32 = 1, kLocalFP64 = 2, kLocalFP32 = 3, kClockKinds = 4 };
@@ -189,6 +201,7 @@ class Context {
   };
   std::vector<PendingLaunch> pending_;
   ClockStat clock_[kClockKinds];
+  std::atomic<long> faults_[kFaultKinds] = {};
 };
 
 double local_algorithmic_bytes(long npw, long count, long m, bool fp64);

@@ -6,9 +6,12 @@ first call.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as ct
 import os
 import subprocess
+import threading
+import weakref
 
 import numpy as np
 
@@ -33,7 +36,8 @@ class DavidsonParams(ct.Structure):
     _fields_ = [("roots", ct.c_long), ("max_iterations", ct.c_long), ("tolerance", ct.c_double),
                 ("subspace_factor", ct.c_long), ("search_precision", ct.c_int), ("search_handoff", ct.c_double),
                 ("stagnation_window", ct.c_int), ("handoff_restart", ct.c_int), ("verbose", ct.c_int),
-                ("timing", ct.c_int), ("band_group_size", ct.c_long)]
+                ("timing", ct.c_int), ("band_group_size", ct.c_long), ("refinement", ct.c_int),
+                ("inner_iterations", ct.c_int)]
 
 
 class DavidsonResult(ct.Structure):
@@ -41,7 +45,10 @@ class DavidsonResult(ct.Structure):
                 ("converged", ct.c_int), ("h_columns_search", ct.c_long), ("h_columns_exact", ct.c_long),
                 ("handoff_iteration", ct.c_long), ("seconds_total", ct.c_double), ("seconds_apply", ct.c_double),
                 ("seconds_gemm", ct.c_double), ("seconds_ritz", ct.c_double), ("seconds_ortho", ct.c_double),
-                ("cholesky_blocks", ct.c_long), ("rr_cholesky", ct.c_long), ("rr_stedc", ct.c_long)]
+                ("cholesky_blocks", ct.c_long), ("rr_cholesky", ct.c_long), ("rr_stedc", ct.c_long),
+                ("rr_barrier_fallbacks", ct.c_long), ("polish_accepted", ct.c_long), ("polish_rejected", ct.c_long),
+                ("manifold_accepted", ct.c_long), ("manifold_rejected", ct.c_long), ("search_rejected", ct.c_long),
+                ("refinement_steps", ct.c_long)]
 
 
 class SnapshotInfo(ct.Structure):
@@ -53,6 +60,34 @@ class SnapshotInfo(ct.Structure):
 
 class HipError(RuntimeError):
     pass
+
+
+# Handles still open, closed in a fixed order when the interpreter exits
+# (services first, so no worker thread is inside a call, then operators, then
+# contexts), before any module or library state is torn down. Nothing relies on
+# finalisers: __del__ only closes what a caller dropped without closing.
+_OPEN = {"services": weakref.WeakSet(), "operators": weakref.WeakSet(), "contexts": weakref.WeakSet()}
+_OPEN_LOCK = threading.Lock()
+
+
+def _track(kind, obj):
+    with _OPEN_LOCK:
+        _OPEN[kind].add(obj)
+
+
+def close_all():
+    """Close every open solve service, operator and context (in that order)."""
+    for kind in ("services", "operators", "contexts"):
+        with _OPEN_LOCK:
+            live = list(_OPEN[kind])
+        for obj in live:
+            try:
+                obj.close()
+            except Exception:  # noqa: BLE001 - teardown continues with the rest
+                pass
+
+
+atexit.register(close_all)
 
 
 def build():
@@ -133,6 +168,7 @@ class Context:
     def __init__(self, device: int = 0):
         self.handle = ct.c_void_p()
         _check(lib().sdft_hip_context_create(ct.c_int(device), ct.byref(self.handle)))
+        _track("contexts", self)
 
     def close(self):
         if self.handle:
@@ -152,6 +188,15 @@ class Context:
 
     def synchronize(self):
         _check(lib().sdft_hip_synchronize(self.handle))
+
+    FAULTS = {"barrier_expiry": 0, "nonfinite_apply": 1}
+
+    def inject_fault(self, kind, count=1):
+        """Make the next `count` events of `kind` misbehave on purpose (tests
+        of failure handling): 'barrier_expiry' (the Rayleigh-Ritz reduction's
+        grid barrier expires; zheevd redoes the solve) or 'nonfinite_apply'
+        (an operator application writes a NaN)."""
+        _check(lib().sdft_hip_inject_fault(self.handle, ct.c_int(self.FAULTS[kind]), ct.c_long(count)))
 
     def hermitian_eigen(self, a, k=None):
         """All eigenvalues (ascending) of the Hermitian matrix a and the lowest k
@@ -280,6 +325,7 @@ class Operator:
         self.ctx = ctx
         if _handle is not None:
             self.handle = _handle
+            _track("operators", self)
             self.n = None
             self._describe()
             return
@@ -314,6 +360,7 @@ class Operator:
             desc.shape[a] = int(shp[a])
         self.handle = ct.c_void_p()
         _check(lib().sdft_hip_operator_create(ctx.handle, ct.byref(desc), ct.byref(self.handle)))
+        _track("operators", self)
         self._keep = None
         self._describe()
 
@@ -416,6 +463,22 @@ class Operator:
                                              _p(init) if init is not None else None, _p(block), ct.byref(cols)))
         return block[:, :cols.value]
 
+    def strict_step(self, kind, x, direction=None, tolerance=1e-12):
+        """One strict certificate-path step on the block x (n x roots) from its
+        fresh FP64 certificate: kind 'polish' (tangent step along `direction`)
+        or 'manifold' (re-Ritz inside the block). Returns (accepted, x,
+        energies, residual norms) of the kept block."""
+        x = np.asfortranarray(x, dtype=np.complex128).copy(order="F")
+        roots = x.shape[1]
+        d = None if direction is None else np.asfortranarray(direction, dtype=np.complex128)
+        e = np.zeros(roots)
+        r = np.zeros(roots)
+        acc = ct.c_int()
+        _check(lib().sdft_hip_strict_step(self.handle, ct.c_int({"polish": 0, "manifold": 1}[kind]),
+                                          ct.c_size_t(roots), ct.c_double(tolerance), _p(x), _p(d), _p(e), _p(r),
+                                          ct.byref(acc)))
+        return bool(acc.value), x, e, r
+
     def close(self):
         if self.handle:
             lib().sdft_hip_operator_destroy(self.handle)
@@ -458,9 +521,10 @@ class Operator:
 
     def davidson(self, roots, max_iterations=256, tolerance=1e-12, initial=None, subspace_factor=8,
                  search_precision=0, search_handoff=1e-8, stagnation_window=3, handoff_restart=1, verbose=0,
-                 timing=0, want_vectors=True, band_group_size=0):
+                 timing=0, want_vectors=True, band_group_size=0, refinement=0, inner_iterations=0):
         p = DavidsonParams(roots, max_iterations, tolerance, subspace_factor, search_precision, search_handoff,
-                           stagnation_window, handoff_restart, verbose, timing, band_group_size)
+                           stagnation_window, handoff_restart, verbose, timing, band_group_size, refinement,
+                           inner_iterations)
         ev = np.zeros(roots)
         res = np.zeros(roots)
         vec = np.zeros((self.n, roots), dtype=np.complex128, order="F") if want_vectors else None
@@ -475,4 +539,8 @@ class Operator:
                     h_columns_exact=r.h_columns_exact, handoff_iteration=r.handoff_iteration,
                     seconds_total=r.seconds_total, seconds_apply=r.seconds_apply, seconds_gemm=r.seconds_gemm,
                     seconds_ritz=r.seconds_ritz, seconds_ortho=r.seconds_ortho,
-                    cholesky_blocks=r.cholesky_blocks, rr_cholesky=r.rr_cholesky, rr_stedc=r.rr_stedc)
+                    cholesky_blocks=r.cholesky_blocks, rr_cholesky=r.rr_cholesky, rr_stedc=r.rr_stedc,
+                    rr_barrier_fallbacks=r.rr_barrier_fallbacks, polish_accepted=r.polish_accepted,
+                    polish_rejected=r.polish_rejected, manifold_accepted=r.manifold_accepted,
+                    manifold_rejected=r.manifold_rejected, search_rejected=r.search_rejected,
+                    refinement_steps=r.refinement_steps)

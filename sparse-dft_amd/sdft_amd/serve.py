@@ -29,8 +29,10 @@ class SolveService:
         self._pending = [0] * workers
         self._lock = threading.Lock()
         self._threads = [threading.Thread(target=self._run, args=(i,), daemon=True) for i in range(workers)]
+        self._closed = False
         for t in self._threads:
             t.start()
+        native._track("services", self)
 
     def context(self, worker):
         return self._contexts[worker]
@@ -59,6 +61,7 @@ class SolveService:
             if job is None:
                 return
             fut, prob, roots, initial, operator, kw, arrival = job
+            op = None
             try:
                 if not fut.set_running_or_notify_cancel():
                     continue
@@ -67,16 +70,23 @@ class SolveService:
                 r = op.davidson(roots, kw.pop("max_iterations", 256), kw.pop("tolerance", 1e-12), initial, **kw)
                 ctx.synchronize()
                 r.update(arrival=arrival, start=start, finish=time.perf_counter(), n=op.n, worker=worker)
-                if operator is None:
-                    op.close()
                 fut.set_result(r)
             except Exception as e:  # noqa: BLE001 - delivered to the caller through the future
                 fut.set_exception(e)
             finally:
+                # an operator built here is released whether or not its solve succeeded
+                if operator is None and op is not None:
+                    op.close()
                 with self._lock:
                     self._pending[worker] -= 1
 
     def close(self):
+        """Stop the workers after the queued solves and release the contexts
+        (operators a caller built on them keep their context alive until they
+        are closed themselves)."""
+        if self._closed:
+            return
+        self._closed = True
         for q in self._queues:
             q.put(None)
         for t in self._threads:
