@@ -82,3 +82,21 @@ def test_reference_band_groups_against_dense(si8, group):
     ref = sl.eigh(h, s, eigvals_only=True, subset_by_index=[0, 15])
     assert np.abs(GOLDEN[f"si8_bandgroups{group}_eigenvalues"] - ref).max() < 1e-10
     assert GOLDEN[f"si8_bandgroups{group}_max_residual"] <= 1e-12
+
+
+LARGE = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "large_golden.npz"))
+
+
+@pytest.mark.parametrize("name,config,kfrac,roots", [("si64", 1, (0., 0., 0.), 160), ("slab0", 3, (0., 0., 0.), 96),
+                                                     ("slab1", 3, (.5, .5, 0.), 96)])
+def test_large_golden_fixture(name, config, kfrac, roots):
+    """Eigenvalues of the full-size workloads written from the reference's
+    compiled Davidson on the oracle operator (tests/golden/make_golden_large.py):
+    the basis size is the one the synthetic recipe gives today, the values are
+    ascending and the reference certified every pair on fresh applications."""
+    c = syn.CONFIGS[config]
+    hkl, _, _ = pyoracle.enumerate_g(c["cell"]().lattice, c["ecut"] * EV + 1e-8, kfrac)
+    assert len(hkl) == int(LARGE[f"{name}_n"])
+    ev = LARGE[f"{name}_eigenvalues"]
+    assert ev.shape == (roots,) and np.all(np.diff(ev) >= 0)
+    assert float(LARGE[f"{name}_max_residual"]) <= 1e-12 and float(LARGE[f"{name}_fresh_residual"]) <= 1e-12

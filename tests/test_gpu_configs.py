@@ -2,6 +2,8 @@
 npw 28817 / 28796 at k = 0 and (1/2, 1/2, 0), 416 projectors) and the Si8
 Monkhorst-Pack 4x4x4 k-points (complex k-phased projectors), including the
 concurrent 64-k-point batch on one GPU."""
+import os
+
 import numpy as np
 import pytest
 
@@ -35,18 +37,49 @@ def fresh_residuals(dev, r):
     return np.linalg.norm(hx - sx * r["eigenvalues"], axis=0)
 
 
-def test_slab_davidson_certified(hip_ctx):
-    """96 roots of the slab at k = 0: FP64 and FP32-search both certify on
-    fresh FP64 applies and agree on every eigenvalue."""
-    prob = syn.config_problem(3)
+LARGE = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "large_golden.npz"))
+
+
+@pytest.mark.parametrize("name,kfrac", [("slab0", (0.0, 0.0, 0.0)), ("slab1", (0.5, 0.5, 0.0))])
+def test_slab_davidson_against_reference(hip_ctx, name, kfrac):
+    """96 roots of the slab at k = 0 and k = (1/2, 1/2, 0): the FP64 path and
+    the FP32-search form (with its Jacobi-Davidson refinement and without)
+    certify on fresh FP64 applies, and every eigenvalue is within 1e-10 Ha of
+    the reference's compiled Davidson on the oracle operator
+    (tests/golden/large_golden.npz)."""
+    prob = syn.config_problem(3, kfrac=kfrac)
     dev = native.Operator(hip_ctx, prob)
     init = syn.initial_psi(prob, 96, 3)
-    r64 = dev.davidson(96, 256, 1e-12, init, subspace_factor=6)
-    r32 = dev.davidson(96, 256, 1e-12, init, subspace_factor=6, search_precision=1, stagnation_window=2)
-    for r in (r64, r32):
+    ref = LARGE[f"{name}_eigenvalues"]
+    runs = [dev.davidson(96, 256, 1e-12, init, subspace_factor=6),
+            dev.davidson(96, 256, 1e-12, init, subspace_factor=6, search_precision=1, stagnation_window=2),
+            dev.davidson(96, 256, 1e-12, init, subspace_factor=6, search_precision=1, stagnation_window=1,
+                         refinement=1)]
+    for r in runs:
         assert r["converged"]
         assert fresh_residuals(dev, r).max() <= 1.01e-12
-    assert np.abs(r64["eigenvalues"] - r32["eigenvalues"]).max() < 1e-10
+        assert np.abs(r["eigenvalues"] - ref).max() < 1e-10
+    dev.close()
+
+
+def test_si64_davidson_against_reference(hip_ctx):
+    """The benchmark workload (Si64 Gamma, 160 roots, the bench's warm block):
+    the FP64 path and the FP32-search form (refinement on and off) certify on
+    fresh FP64 applies and match the reference's compiled Davidson on the
+    oracle operator within 1e-10 Ha for every eigenvalue."""
+    prob = syn.config_problem(1)
+    dev = native.Operator(hip_ctx, prob)
+    init = syn.initial_psi(prob, 160, 1)
+    ref = LARGE["si64_eigenvalues"]
+    runs = [dev.davidson(160, 256, 1e-12, init, subspace_factor=5),
+            dev.davidson(160, 256, 1e-12, init, subspace_factor=5, search_precision=1, stagnation_window=1),
+            dev.davidson(160, 256, 1e-12, init, subspace_factor=5, search_precision=1, stagnation_window=1,
+                         refinement=1)]
+    for r in runs:
+        assert r["converged"]
+        assert fresh_residuals(dev, r).max() <= 1.01e-12
+        assert np.abs(r["eigenvalues"] - ref).max() < 1e-10
+    dev.close()
 
 
 @pytest.mark.parametrize("k", [0, 21, 63])
