@@ -121,6 +121,12 @@ def main():
     ap.add_argument("--handoff-restart", type=int, default=1,
                     help="1: thick-restart to the 2 x roots Ritz vectors at the FP32 -> FP64 hand-off; "
                          "0: refresh the whole subspace in FP64")
+    ap.add_argument("--refinement", type=int, default=1,
+                    help="FP32 search after the hand-off: 1 = block Jacobi-Davidson refinement (FP32 CG correction "
+                         "solves, FP64 images and Rayleigh-Ritz); 0 = the reference's FP64 Davidson iterations")
+    ap.add_argument("--inner-iterations", type=int, default=6, help="CG steps per refinement correction solve")
+    ap.add_argument("--search-handoff", type=float, default=1e-8,
+                    help="max residual at which the FP32 search hands off to FP64 (the reference's 1e-8)")
     ap.add_argument("--serving", type=int, default=0,
                     help="also run N serving requests (Si8..Si64 mix, Poisson arrivals) and report their p50/p99")
     ap.add_argument("--serving-workers", type=int, default=4)
@@ -153,7 +159,9 @@ def main():
         t0 = time.perf_counter()
         r = op.davidson(ROOTS, 256, 1e-12, init, search_precision=precision, verbose=args.verbose,
                         timing=args.timing, subspace_factor=args.subspace_factor,
-                        stagnation_window=args.stagnation_window, handoff_restart=args.handoff_restart)
+                        stagnation_window=args.stagnation_window, handoff_restart=args.handoff_restart,
+                        refinement=args.refinement if precision else 0, inner_iterations=args.inner_iterations,
+                        search_handoff=args.search_handoff)
         ctx.synchronize()
         r["wall"] = time.perf_counter() - t0
         if not r["converged"]:
@@ -170,6 +178,7 @@ def main():
         runs.append(r)
     clock = ctx.kernel_clock_stats()
     ctx.kernel_clock(False)
+    solve(0)  # FP64 warm-up: the FP64 path's buffers are allocated outside its timed solves too
     fp64 = [solve(0) for _ in range(max(1, args.fp64_steps))]
     fp64_wall = float(np.median([r["wall"] for r in fp64]))
     total = float(np.sum(lat))

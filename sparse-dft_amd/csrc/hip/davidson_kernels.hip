@@ -315,6 +315,66 @@ dim3 grid_cols(long n, long cols) {
   if (bx > 64) bx = 64;
   return dim3(unsigned(bx), unsigned(cols));
 }
+
+// ---- block preconditioned CG of the Jacobi-Davidson correction equations
+// (one independent CG per column; its scalars live in device arrays, so an
+// iteration needs no host round trip)
+
+// out(:, c) = in(:, src[c])
+__global__ void k_gather_columns(const double* __restrict__ in, long ldi, const int* __restrict__ src, long n,
+                                 double* __restrict__ out, long ldo) {
+  const long col = blockIdx.y;
+  const long sc = src[col];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    out[2 * (i + col * ldo)] = in[2 * (i + sc * ldi)];
+    out[2 * (i + col * ldo) + 1] = in[2 * (i + sc * ldi) + 1];
+  }
+}
+
+// z(:, c) = K(kinetic_i, wk[src[c]]) r(:, c): the kinetic preconditioner as a
+// positive diagonal (the Davidson correction is its negative)
+__global__ void k_jd_precondition(const double* __restrict__ r, long ldr, const int* __restrict__ src,
+                                  const double* __restrict__ wk, const double* __restrict__ kinetic, long n,
+                                  double* __restrict__ z, long ldz) {
+  const long col = blockIdx.y;
+  const double w = wk[src[col]];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double k = kinetic_preconditioner(kinetic[i], w);
+    z[2 * (i + col * ldz)] = r[2 * (i + col * ldr)] * k;
+    z[2 * (i + col * ldz) + 1] = r[2 * (i + col * ldr) + 1] * k;
+  }
+}
+
+// alpha = rho / (p, q) (0 when the curvature is not positive and finite:
+// that column stops moving); t += alpha p; r -= alpha q
+__global__ void k_cg_update(double* __restrict__ t, double* __restrict__ r, const double* __restrict__ p,
+                            const double* __restrict__ q, long ld, long n, const double* __restrict__ rho,
+                            const double* __restrict__ pq) {
+  const long col = blockIdx.y;
+  const double c = pq[col], num = rho[col];
+  const double a = (c > 0. && isfinite(c) && isfinite(num)) ? num / c : 0.;
+  if (a == 0.) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long k = 2 * (i + col * ld);
+    t[k] += a * p[k];
+    t[k + 1] += a * p[k + 1];
+    r[k] -= a * q[k];
+    r[k + 1] -= a * q[k + 1];
+  }
+}
+
+// beta = rho_new / rho_old (0 when rho_old is not positive); p = z + beta p
+__global__ void k_cg_direction(double* __restrict__ p, const double* __restrict__ z, long ld, long n,
+                               const double* __restrict__ rho_new, const double* __restrict__ rho_old) {
+  const long col = blockIdx.y;
+  const double o = rho_old[col], v = rho_new[col];
+  const double b = (o > 0. && isfinite(o) && isfinite(v)) ? v / o : 0.;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long k = 2 * (i + col * ld);
+    p[k] = z[k] + b * p[k];
+    p[k + 1] = z[k + 1] + b * p[k + 1];
+  }
+}
 }  // namespace
 
 // Davidson seed block (src/pw_core.cpp:1027-1112) on the device, bit-equal to
@@ -426,6 +486,27 @@ hipError_t launch_precondition(const double* r, long ldr, const int* src, long c
   if (cols <= 0) return hipSuccess;
   SDFT_CHECKED(hipLaunchKernelGGL(k_precondition, grid_cols(n, cols), dim3(256), 0, s, r, ldr, src, wk, kinetic, n,
                                   out, ldo));
+}
+hipError_t launch_gather_columns(const double* in, long ldi, const int* src, long n, long cols, double* out, long ldo,
+                                 hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_gather_columns, grid_cols(n, cols), dim3(256), 0, s, in, ldi, src, n, out, ldo));
+}
+hipError_t launch_jd_precondition(const double* r, long ldr, const int* src, long cols, const double* wk,
+                                  const double* kinetic, long n, double* z, long ldz, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_jd_precondition, grid_cols(n, cols), dim3(256), 0, s, r, ldr, src, wk, kinetic, n,
+                                  z, ldz));
+}
+hipError_t launch_cg_update(double* t, double* r, const double* p, const double* q, long ld, long n, long cols,
+                            const double* rho, const double* pq, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_cg_update, grid_cols(n, cols), dim3(256), 0, s, t, r, p, q, ld, n, rho, pq));
+}
+hipError_t launch_cg_direction(double* p, const double* z, long ld, long n, long cols, const double* rho_new,
+                               const double* rho_old, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_cg_direction, grid_cols(n, cols), dim3(256), 0, s, p, z, ld, n, rho_new, rho_old));
 }
 hipError_t launch_ritz_residual(const double* hx, const double* sx, long ld, long n, long cols, const double* values,
                                 double* r, hipStream_t s) {

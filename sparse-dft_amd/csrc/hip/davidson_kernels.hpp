@@ -31,6 +31,20 @@ hipError_t launch_polish_update(const double* x, const double* d, long ld, long 
                                 double* trial, hipStream_t s);
 hipError_t launch_hermitise(const double* m, long ldm, long size, double* out, long ldo, hipStream_t s);
 hipError_t launch_inv_sqrt_columns(double* u, long ld, long rows, long cols, const double* lambda, hipStream_t s);
+// Block preconditioned CG of the Jacobi-Davidson correction equations (one
+// CG per column, scalars in device arrays):
+// out(:, c) = in(:, src[c])
+hipError_t launch_gather_columns(const double* in, long ldi, const int* src, long n, long cols, double* out, long ldo,
+                                 hipStream_t s);
+// z(:, c) = K(kinetic, wk[src[c]]) r(:, c), K the kinetic preconditioner (positive)
+hipError_t launch_jd_precondition(const double* r, long ldr, const int* src, long cols, const double* wk,
+                                  const double* kinetic, long n, double* z, long ldz, hipStream_t s);
+// a = rho / pq per column (0 unless pq > 0 and finite); t += a p; r -= a q
+hipError_t launch_cg_update(double* t, double* r, const double* p, const double* q, long ld, long n, long cols,
+                            const double* rho, const double* pq, hipStream_t s);
+// b = rho_new / rho_old per column (0 unless rho_old > 0); p = z + b p
+hipError_t launch_cg_direction(double* p, const double* z, long ld, long n, long cols, const double* rho_new,
+                               const double* rho_old, hipStream_t s);
 hipError_t launch_finite_check(const double* x, long ld, long n, long cols, int* bad, hipStream_t s);
 // max |B - I| (as the bits of a non-negative double, max-folded into *max_bits,
 // which the caller zeroes); E = B - I written when e is not null

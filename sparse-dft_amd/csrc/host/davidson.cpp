@@ -310,6 +310,8 @@ class Solver {
     projected_size_ = 0;
   }
   bool polish_step(double* direction);
+  bool refine(long& it, long& iterations);
+  void correction_cg(const std::vector<int>& active, double* t);
   bool manifold_step();
   void preserved_subspace(long keep, double* retained);
 
@@ -675,6 +677,190 @@ void Solver::preserved_subspace(long keep, double* retained) {
   gemm(true, 0, 0, n_, keep - R_, m, 1.0, basis_.p, n_, q + 2 * R_ * m, m, retained + 2 * R_ * n_, n_);
 }
 
+// Jacobi-Davidson refinement after the hand-off from search arithmetic. The
+// block W = [X, G] holds the R wanted Ritz vectors X and R guard Ritz vectors
+// G (the next ones up), columns 0..2R of basis_ with their images in hb_ and
+// sb_. Each step:
+//   1. fresh FP64 H and S images of W; Rayleigh quotients, S normalisation
+//      and residuals of X; when every root meets the tolerance, certify();
+//   2. corrections T for the unconverged roots from the correction equations
+//      P^H (H - theta_j S) P t_j = r_j, P = I - X (S X)^H, by preconditioned
+//      CG in search (FP32) arithmetic (correction_cg);
+//   3. T projected against W and S-orthonormalised in FP64, with FP64 images;
+//   4. FP64 Rayleigh-Ritz of [W, T]; the lowest 2R Ritz vectors are the next W.
+// The guards enter the Rayleigh-Ritz (roots can move across the block edge)
+// but not the projector: guards from the search subspace are far less
+// accurate than the wanted roots, and projecting them out distorts the
+// correction equations of the roots at the top of the block. Returns true when certified; false (x_ holding the current X, W in
+// retained_) when a step could not be formed or the residual stopped
+// improving, for the FP64 Davidson iterations to continue from.
+
+// Block preconditioned CG for the active roots (see refine); t: n x |active|.
+// The projector is the one of the wanted block X (the first R columns of W);
+// the guards enter through the Rayleigh-Ritz only. Applications and GEMMs run
+// in FP32 arithmetic whatever the solver's current precision.
+void Solver::correction_cg(const std::vector<int>& active, double* t) {
+  const long n = n_, R = R_, cols = long(active.size()), k = R;
+  const Precision f32 = Precision::FP32;
+  const int steps = p_.inner_iterations > 0 ? p_.inner_iterations : 6;
+  const double* w = basis_.p;
+  const double* sw = sb_.p;
+  double* p = w3_.p;
+  double* q = p + 2 * n * R;
+  double* hq = q + 2 * n * R;
+  double* sq = hq + 2 * n * R;
+  double* z = w4_.p;
+  double* res = z + 2 * n * R;
+  double* coef = m1_.p;  // 2R x cols
+  double* base = small_.p + 8 + 4 * R;
+  double* theta = base;
+  double* wk = base + R;
+  double* rho0 = base + 2 * R;
+  double* rho1 = base + 3 * R;
+  double* pq = base + 4 * R;
+  int* src = reinterpret_cast<int*>(base + 5 * R);
+  std::vector<double> th(cols);
+  for (long c = 0; c < cols; ++c) th[c] = values_[active[c]];
+  upload(theta, th.data(), cols);
+  upload_bytes(src, active.data(), sizeof(int) * cols);
+  check(launch_kinetic_weights(w, n, n, R, op_.kinetic_device(), wk, s_), "kinetic weights");
+  check(launch_gather_columns(res_.p, n, src, n, cols, res, n, s_), "gather");
+  // P y = y - X (SX)^H y and P^H y = y - SX X^H y in search arithmetic
+  auto project = [&](double* y, bool adjoint) {
+    ctx_.zgemm(f32, 1, 0, k, cols, n, 1.0, adjoint ? w : sw, n, y, n, coef, k, nullptr, 0, 0.0);
+    ctx_.zgemm(f32, 0, 0, n, cols, k, -1.0, adjoint ? sw : w, n, coef, k, y, n, y, n, 1.0);
+  };
+  project(res, true);
+  check(hipMemsetAsync(t, 0, 16 * std::size_t(n) * cols, s_), "zero");
+  check(launch_jd_precondition(res, n, src, cols, wk, op_.kinetic_device(), n, z, n, s_), "precondition");
+  project(z, false);
+  copy(p, z, n * cols);
+  check(launch_real_dots(res, n, z, n, n, cols, rho0, s_), "dots");
+  for (int it = 0; it < steps; ++it) {
+    op_.apply(2, p, cols, hq, sq, f32);
+    stats_.h_columns_search += cols;
+    check(launch_fma_residual(hq, sq, n, n, cols, theta, q, s_), "shifted");
+    project(q, true);
+    check(launch_real_dots(p, n, q, n, n, cols, pq, s_), "dots");
+    check(launch_cg_update(t, res, p, q, n, n, cols, rho0, pq, s_), "cg update");
+    if (p_.verbose > 1) {
+      const std::vector<double> cur = col_norms(res, n, cols);
+      const std::vector<double> curv = to_host(pq, cols);
+      double worst = 0, neg = 0;
+      for (long c = 0; c < cols; ++c) {
+        worst = std::max(worst, cur[c] / std::max(res_norms_[active[c]], 1e-300));
+        neg += curv[c] <= 0;
+      }
+      std::fprintf(stderr, "  cg step %d: max relative residual %.3e, nonpositive curvature %g of %ld\n", it, worst, neg,
+                   cols);
+    }
+    if (it + 1 == steps) break;
+    check(launch_jd_precondition(res, n, src, cols, wk, op_.kinetic_device(), n, z, n, s_), "precondition");
+    project(z, false);
+    check(launch_real_dots(res, n, z, n, n, cols, rho1, s_), "dots");
+    check(launch_cg_direction(p, z, n, n, cols, rho1, rho0, s_), "cg direction");
+    std::swap(rho0, rho1);
+  }
+}
+
+// W (2R columns, n x 2R) in retained_ on entry and exit.
+bool Solver::refine(long& it, long& iterations) {
+  const long n = n_, R = R_, k = 2 * R;
+  const double t0 = now();
+  double best = std::numeric_limits<double>::infinity();
+  int stalls = 0;
+  double* wbuf = retained_.p;
+  for (; it + 1 < p_.max_iterations;) {
+    ++it;
+    iterations = it + 1;
+    // 1. fresh FP64 images of W; X = its first R columns
+    copy(basis_.p, wbuf, n * k);
+    apply(2, basis_.p, k, hb_.p, sb_.p, true);
+    const std::vector<double> metric = real_dots(basis_.p, sb_.p, R);
+    const std::vector<double> energy = real_dots(basis_.p, hb_.p, R);
+    std::vector<double> root(R);
+    for (long r = 0; r < R; ++r) {
+      if (!(metric[r] > 0) || !std::isfinite(energy[r])) throw std::runtime_error("HIP: invalid refinement Ritz metric");
+      values_[r] = energy[r] / metric[r];
+      root[r] = std::sqrt(metric[r]);
+    }
+    upload(small_.p + 8, root.data(), R);
+    for (DevBuf<double>* b : {&basis_, &hb_, &sb_})
+      check(launch_divide_columns(b->p, n, n, R, small_.p + 8, s_), "normalise");
+    upload(small_.p + 8 + R, values_.data(), R);
+    check(launch_fma_residual(hb_.p, sb_.p, n, n, R, small_.p + 8 + R, res_.p, s_), "residual");
+    refresh_norms();
+    stats_.max_residual_history.push_back(max_res());
+    if (search_residual_ >= 0) {
+      if (max_res() > std::max(8 * tol_, 10 * search_residual_)) ++stats_.search_rejected;
+      search_residual_ = -1;
+    }
+    if (p_.verbose)
+      std::fprintf(stderr, "davidson it=%ld refinement max_residual=%.3e t=%.1fms\n", it + 1, max_res(),
+                   1e3 * (now() - t0));
+    if (converged_) {
+      copy(x_.p, basis_.p, n * R);
+      certify();
+      if (converged_) return true;
+    }
+    if (max_res() < 0.5 * best) {
+      best = max_res();
+      stalls = 0;
+    } else if (++stalls >= 2) {
+      copy(x_.p, basis_.p, n * R);
+      return false;
+    }
+    std::vector<int> active;
+    for (long r = 0; r < R; ++r)
+      if (res_norms_[r] > .5 * tol_) active.push_back(int(r));
+    const long cols = long(active.size());
+    ++stats_.refinement_steps;
+    // 2. corrections in search arithmetic
+    double* t = w4_.p + 4 * n * R;
+    correction_cg(active, t);
+    // 3. T projected against W and S-orthonormalised, with FP64 images
+    size_ = k;
+    double* v = w3_.p;
+    double* metric_v = w3_.p + 2 * n * R;
+    double* coef = m1_.p;
+    gemm(true, 1, 0, k, cols, n, 1.0, sb_.p, n, t, n, coef, k);
+    gemm(true, 0, 0, n, cols, k, -1.0, basis_.p, n, coef, k, v, n, t, n, 1.0);
+    apply(1, v, cols, nullptr, metric_v, true);
+    double* gram = m3_.p;
+    double* gh = m3_.p + 2 * cols * cols;
+    gemm(true, 1, 0, cols, cols, n, 1.0, v, n, metric_v, n, gram, cols);
+    check(launch_hermitise(gram, cols, cols, gh, cols, s_), "hermitise");
+    long added = cols;
+    if (!cholesky_normalise(v, metric_v, gram, gh, cols, true)) {
+      // nearly dependent corrections (a correction inside span(W) up to its
+      // search-arithmetic rounding): the reference's block CGS2 with its
+      // eigen-normalisation and dependency test keeps the independent part
+      copy(corr_.p, v, n * cols);
+      append_block(corr_.p, cols);
+      added = size_ - k;
+      if (added <= 0) {
+        copy(x_.p, basis_.p, n * R);
+        return false;
+      }
+    }
+    apply(0, col(basis_, k, n), added, col(hb_, k, n), nullptr, true);
+    size_ = k + added;
+    // 4. Rayleigh-Ritz of [W, T]
+    const long m = size_;
+    double* A = rr_.p;
+    double* B = rrb_.p;
+    gemm(true, 1, 0, m, m, n, 1.0, basis_.p, n, hb_.p, n, ch_.p, m);
+    gemm(true, 1, 0, m, m, n, 1.0, basis_.p, n, sb_.p, n, cs_.p, m);
+    check(launch_hermitise(ch_.p, m, m, A, m, s_), "hermitise");
+    check(launch_hermitise(cs_.p, m, m, B, m, s_), "hermitise");
+    const std::vector<double> w = rr_solve(A, B, m, k);
+    for (long r = 0; r < R; ++r) values_[r] = w[r];
+    gemm(true, 0, 0, n, k, m, 1.0, basis_.p, n, A, m, wbuf, n);
+  }
+  copy(x_.p, wbuf, n * R);
+  return false;
+}
+
 StrictStepOutput Solver::strict_primitive(int kind, const C* x, const C* direction) {
   const long n = n_, R = R_;
   ctx_.upload_bulk(x_.p, x, 16 * std::size_t(n) * R);
@@ -759,7 +945,8 @@ DavidsonOutput Solver::run(const C* initial) {
       search_residual_ = -1;
     }
     if (p_.verbose)
-      std::fprintf(stderr, "davidson it=%ld size=%ld exact=%d max_residual=%.3e\n", it + 1, size_, int(exact_), max_res());
+      std::fprintf(stderr, "davidson it=%ld size=%ld exact=%d max_residual=%.3e t=%.1fms\n", it + 1, size_, int(exact_),
+                   max_res(), 1e3 * (now() - t_start));
     if (!exact_) {
       // FP32 search hand-off: the reference's 1e-8 threshold, or a stalled
       // FP32 residual (its rounding floor lies far above 1e-8)
@@ -770,6 +957,23 @@ DavidsonOutput Solver::run(const C* initial) {
       if (max_res() < p_.search_handoff || it + 1 == p_.max_iterations || stalled) {
         stats_.handoff_iteration = it + 1;
         search_residual_ = max_res();
+        if (p_.refinement && cap_ >= 4 * R && size_ >= 2 * R && p_.handoff_restart >= 1 &&
+            it + 1 < p_.max_iterations) {
+          // the lowest 2R Ritz vectors of the search subspace in FP64
+          // arithmetic, then the Jacobi-Davidson refinement; it returns false
+          // to hand them to the FP64 Davidson iterations below
+          exact_ = true;
+          gemm(true, 0, 0, n, 2 * R, size_, 1.0, basis_.p, n, A, size_, retained.p, n);
+          if (refine(it, iterations)) {
+            certified = true;
+            break;
+          }
+          size_ = 0;
+          computed_h_ = 0;
+          projected_size_ = 0;
+          append_block(retained.p, 2 * R);
+          continue;
+        }
         const long keep = std::min(size_, long(std::max(2, p_.handoff_restart)) * R);
         if (p_.handoff_restart && size_ > keep) {
           // keep the lowest Ritz vectors (FP64 rotation), then the exact block
