@@ -74,7 +74,9 @@ def test_si64_davidson_against_reference(hip_ctx):
     runs = [dev.davidson(160, 256, 1e-12, init, subspace_factor=5),
             dev.davidson(160, 256, 1e-12, init, subspace_factor=5, search_precision=1, stagnation_window=1),
             dev.davidson(160, 256, 1e-12, init, subspace_factor=5, search_precision=1, stagnation_window=1,
-                         refinement=1)]
+                         refinement=1),
+            dev.davidson(160, 256, 1e-12, init, subspace_factor=5, search_precision=1, refinement=1,
+                         search_handoff=3e-3)]
     for r in runs:
         assert r["converged"]
         assert fresh_residuals(dev, r).max() <= 1.01e-12

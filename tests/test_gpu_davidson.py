@@ -30,11 +30,17 @@ def certify(h, s, r, roots):
     return res, np.abs(gram - np.eye(roots)).max()
 
 
-@pytest.mark.parametrize("search_precision", [0, 1])
-def test_davidson_si8_certified(si8, si8_dev, si8_dense, search_precision):
+@pytest.mark.parametrize("search_precision,refinement,handoff", [(0, 0, 1e-8), (1, 0, 1e-8), (1, 1, 1e-8),
+                                                                   (1, 1, 3e-3)])
+def test_davidson_si8_certified(si8, si8_dev, si8_dense, search_precision, refinement, handoff):
+    """FP64 path, FP32 search with the reference's FP64 Davidson tail, and FP32
+    search with the Jacobi-Davidson refinement (hand-off at 1e-8 and 3e-3)."""
     h, s, w = si8_dense
     init = syn.initial_psi(si8, 32, 0)
-    r = si8_dev.davidson(16, 256, 1e-12, init, search_precision=search_precision)
+    r = si8_dev.davidson(16, 256, 1e-12, init, search_precision=search_precision, refinement=refinement,
+                         search_handoff=handoff)
+    if refinement:
+        assert r["refinement_steps"] > 0
     assert r["converged"], r
     assert np.abs(r["eigenvalues"] - w[:16]).max() < 1e-10
     res, gerr = certify(h, s, r, 16)
