@@ -24,6 +24,7 @@ extern "C" {
 
 typedef struct sdft_hip_context sdft_hip_context;
 typedef struct sdft_hip_operator sdft_hip_operator;
+typedef struct sdft_hip_batch sdft_hip_batch;
 
 #define SDFT_HIP_OK 0
 #define SDFT_HIP_RUNTIME_ERROR 1
@@ -311,6 +312,34 @@ int sdft_hip_tridiagonal_eigen(sdft_hip_context* ctx, size_t n, const double* d,
 
 int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* params, const double* initial,
                       sdft_hip_davidson_result* result);
+
+/* Batched k-point application (no counterpart as one call in the reference,
+ * whose k-points are independent PlaneWaveOperator applications on OpenMP
+ * teams, src/scf_core.cpp:3051-3061): H psi and S psi of `count` device
+ * operators of one context that share an FFT grid shape (the k-points of one
+ * crystal; uneven plane-wave counts, k-dependent projectors) for `columns`
+ * columns each, as ONE persistent kernel launch over (operator, column) work
+ * items, captured once in a HIP graph and replayed. The launch shape (threads,
+ * xy-planes per LDS round, workgroups) is chosen at creation by timing the
+ * candidates on the device, and cached per problem shape for the process.
+ * Operators need atom-block projectors; the operators must outlive the
+ * batch. precision 0 FP64, 1 FP32 arithmetic. */
+int sdft_hip_batch_create(sdft_hip_context* ctx, size_t count, sdft_hip_operator* const* ops, size_t columns,
+                          int precision, sdft_hip_batch** out);
+int sdft_hip_batch_destroy(sdft_hip_batch* batch);
+/* offsets[k] (count + 1 entries, complex elements): start of operator k's
+ * n_k x columns column-major block in the concatenated input and outputs. */
+int sdft_hip_batch_offsets(sdft_hip_batch* batch, long* offsets);
+/* Host pointers, concatenated blocks: uploads x, replays the graph, returns
+ * H x and S x. */
+int sdft_hip_batch_apply(sdft_hip_batch* batch, const double* x, double* hy, double* sy);
+/* Mean device time of one graph replay over `replays` replays (HIP events,
+ * inputs resident). */
+int sdft_hip_batch_benchmark(sdft_hip_batch* batch, int replays, double* milliseconds);
+/* The tuned launch shape and its measured time; *from_cache = 1 when it came
+ * from an earlier batch of the same shape. Any output may be NULL. */
+int sdft_hip_batch_plan(sdft_hip_batch* batch, int* threads, int* planes, int* groups, double* tuned_ms,
+                        int* from_cache);
 
 /* One strict certificate-path step on a given block (the reference's
  * strict_residual_polish_step and strict_root_manifold_step primitives,

@@ -19,35 +19,13 @@
 #include <cstdlib>
 
 #include "fft_device.hpp"
+#include "fft_dispatch.hpp"
 #include "fft_lengths.hpp"
 #include "local_h.hpp"
 
 namespace sdft_hip {
 
-// LDS stride of contiguous lines (z-sticks, plane rows): odd, so that threads
-// one line apart hit different LDS banks.
-__host__ __device__ inline int line_stride(int n) { return n | 1; }
-
-
 namespace {
-
-template <typename T, int S, bool TO_SLOTS>
-__device__ __forceinline__ void line_fft_dispatch(int length, cx<T>* base, const LineGeom& g, const cx<T>* tw, int tid,
-                                                  int nt) {
-  switch (length) {
-#define SDFT_HIP_CASE(L, A, B)                                 \
-  case L:                                                      \
-    if (TO_SLOTS)                                              \
-      lds_fft_to_slots<A, B, S, T>(base, g, tw, tid, nt);      \
-    else                                                       \
-      lds_fft_from_slots<A, B, S, T>(base, g, tw, tid, nt);    \
-    break;
-    SDFT_HIP_FOR_EACH_LENGTH(SDFT_HIP_CASE)
-#undef SDFT_HIP_CASE
-    default:
-      break;
-  }
-}
 
 template <typename T>
 __device__ __forceinline__ cx<T> load_cx(const double* p) {

@@ -15,6 +15,7 @@
 
 #include "../hip/davidson_kernels.hpp"
 #include "../hip/tridiag_eig.hpp"
+#include "batch.hpp"
 #include "davidson.hpp"
 #include "dense_eigen.hpp"
 #include "device.hpp"
@@ -54,6 +55,13 @@ struct sdft_hip_operator {
   sdft_hip::DeviceOperator impl;
   sdft_hip_operator(sdft_hip_context* c, const sdft_hip::OperatorDesc& d) : owner(c), impl(c->impl, d) {}
   sdft_hip_operator(sdft_hip_context* c, const sdft_hip::SnapshotArrays& s) : owner(c), impl(c->impl, s) {}
+};
+struct sdft_hip_batch {
+  ContextRef owner;
+  sdft_hip::OperatorBatch impl;
+  sdft_hip_batch(sdft_hip_context* c, const std::vector<sdft_hip::DeviceOperator*>& ops, long columns,
+                 sdft_hip::Precision prec)
+      : owner(c), impl(c->impl, ops, columns, prec) {}
 };
 
 namespace {
@@ -582,6 +590,65 @@ int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* par
     result->manifold_rejected = out.stats.manifold_rejected;
     result->search_rejected = out.stats.search_rejected;
     result->refinement_steps = out.stats.refinement_steps;
+  });
+}
+
+int sdft_hip_batch_create(sdft_hip_context* ctx, size_t count, sdft_hip_operator* const* ops, size_t columns,
+                          int precision, sdft_hip_batch** out) {
+  return guarded([&] {
+    if (!ctx || !ops || !count || !columns || !out) throw std::invalid_argument("HIP: invalid operator batch");
+    std::vector<sdft_hip::DeviceOperator*> list;
+    for (size_t k = 0; k < count; ++k) {
+      if (!ops[k] || ops[k]->owner.ctx != ctx) throw std::invalid_argument("HIP: batch operators must share the context");
+      list.push_back(&ops[k]->impl);
+    }
+    check_int_range({columns, count});
+    *out = new sdft_hip_batch(ctx, list, long(columns), to_precision(precision));
+  });
+}
+
+int sdft_hip_batch_destroy(sdft_hip_batch* batch) {
+  return guarded([&] { delete batch; });
+}
+
+int sdft_hip_batch_offsets(sdft_hip_batch* batch, long* offsets) {
+  return guarded([&] {
+    if (!batch || !offsets) throw std::invalid_argument("HIP: invalid batch layout output");
+    const auto& o = batch->impl.offsets();
+    std::copy(o.begin(), o.end(), offsets);
+  });
+}
+
+int sdft_hip_batch_apply(sdft_hip_batch* batch, const double* x, double* hy, double* sy) {
+  return guarded([&] {
+    if (!batch || !x || !hy || !sy) throw std::invalid_argument("HIP: invalid batch buffers");
+    auto& b = batch->impl;
+    auto& cx = batch->owner.ctx->impl;
+    const std::size_t bytes = 16 * std::size_t(b.offsets().back());
+    cx.upload_bulk(b.input(), x, bytes);
+    b.replay();
+    cx.download_bulk(hy, b.output_h(), bytes);
+    cx.download_bulk(sy, b.output_s(), bytes);
+  });
+}
+
+int sdft_hip_batch_benchmark(sdft_hip_batch* batch, int replays, double* milliseconds) {
+  return guarded([&] {
+    if (!batch || !milliseconds) throw std::invalid_argument("HIP: invalid batch benchmark output");
+    *milliseconds = batch->impl.benchmark(replays);
+  });
+}
+
+int sdft_hip_batch_plan(sdft_hip_batch* batch, int* threads, int* planes, int* groups, double* tuned_ms,
+                        int* from_cache) {
+  return guarded([&] {
+    if (!batch) throw std::invalid_argument("HIP: null batch");
+    const auto& l = batch->impl.plan();
+    if (threads) *threads = l.threads;
+    if (planes) *planes = l.planes;
+    if (groups) *groups = l.groups;
+    if (tuned_ms) *tuned_ms = batch->impl.tuned_ms();
+    if (from_cache) *from_cache = batch->impl.plan_from_cache() ? 1 : 0;
   });
 }
 

@@ -284,6 +284,20 @@ class DeviceOperator {
   std::shared_ptr<void>& davidson_cache() { return davidson_cache_; }
   std::size_t max_chunk() const { return chunk_; }
   const GridPlanDev& grid_plan() const { return grid_.dev; }
+  // device arrays of the nonlocal part (atom-block form 3): projectors
+  // np x n column-major (FP64 and FP32 copies), Hermitised per-atom D and Q
+  // blocks (column-major, at block_offset per atom), first projector and
+  // projector count per atom
+  int nonlocal_form() const { return form_; }
+  int atoms() const { return atoms_; }
+  const double* projectors64() const { return b64_.p; }
+  const float* projectors32() const { return b32_.p; }
+  const double* onsite_d_blocks() const { return dblk_.p; }
+  const double* onsite_q_blocks() const { return qblk_.p; }
+  const long* block_offsets() const { return block_offset_.p; }
+  const int* atom_offsets() const { return atom_offset_.p; }
+  const int* atom_row_counts() const { return atom_rows_.p; }
+  const std::vector<int>& atom_rows_host() const { return atom_rows_host_; }
   // stick buffer 0 or 1 sized for `columns` columns at precision `prec`
   void* stick_buffer(int which, std::size_t columns, Precision prec);
 

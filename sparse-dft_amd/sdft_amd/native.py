@@ -544,3 +544,58 @@ class Operator:
                     polish_rejected=r.polish_rejected, manifold_accepted=r.manifold_accepted,
                     manifold_rejected=r.manifold_rejected, search_rejected=r.search_rejected,
                     refinement_steps=r.refinement_steps)
+
+
+class Batch:
+    """H psi and S psi of many device operators of one context that share an
+    FFT grid (the k-points of one crystal) as one persistent kernel launch
+    replayed from a HIP graph; the launch shape is autotuned on creation and
+    cached per problem shape."""
+
+    def __init__(self, ctx: Context, operators, columns, precision=0):
+        self.ctx = ctx
+        self.operators = list(operators)  # kept alive while the batch exists
+        arr = (ct.c_void_p * len(self.operators))(*[op.handle.value for op in self.operators])
+        self.handle = ct.c_void_p()
+        self.columns = int(columns)
+        _check(lib().sdft_hip_batch_create(ctx.handle, ct.c_size_t(len(self.operators)), arr,
+                                           ct.c_size_t(self.columns), ct.c_int(precision), ct.byref(self.handle)))
+        _track("operators", self)
+        off = np.zeros(len(self.operators) + 1, dtype=np.int64)
+        _check(lib().sdft_hip_batch_offsets(self.handle, _p(off, _lp)))
+        self.offsets = off
+
+    def apply(self, blocks):
+        """blocks: one n_k x columns array per operator; returns (H blocks, S blocks)."""
+        x = np.zeros(int(self.offsets[-1]), dtype=np.complex128)
+        for k, b in enumerate(blocks):
+            x[self.offsets[k]:self.offsets[k + 1]] = np.asfortranarray(b, dtype=np.complex128).ravel(order="F")
+        hy = np.zeros_like(x)
+        sy = np.zeros_like(x)
+        _check(lib().sdft_hip_batch_apply(self.handle, _p(x), _p(hy), _p(sy)))
+        split = lambda a: [a[self.offsets[k]:self.offsets[k + 1]].reshape(self.columns, -1).T
+                           for k in range(len(self.operators))]
+        return split(hy), split(sy)
+
+    def benchmark(self, replays=20):
+        ms = ct.c_double()
+        _check(lib().sdft_hip_batch_benchmark(self.handle, ct.c_int(replays), ct.byref(ms)))
+        return ms.value
+
+    def plan(self):
+        t, p, g, c = ct.c_int(), ct.c_int(), ct.c_int(), ct.c_int()
+        ms = ct.c_double()
+        _check(lib().sdft_hip_batch_plan(self.handle, ct.byref(t), ct.byref(p), ct.byref(g), ct.byref(ms),
+                                         ct.byref(c)))
+        return dict(threads=t.value, planes=p.value, groups=g.value, tuned_ms=ms.value, from_cache=bool(c.value))
+
+    def close(self):
+        if self.handle:
+            lib().sdft_hip_batch_destroy(self.handle)
+            self.handle = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
