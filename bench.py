@@ -129,9 +129,11 @@ def main():
                     help="max residual at which the FP32 search hands off (the reference hands off at 1e-8; with the "
                          "Jacobi-Davidson refinement 3e-3 measured fastest: 1e-2 / 3e-3 / 1e-3 / 1e-4 / 1e-8 gave "
                          "621 / 621 / 591-620 / 555 / 535 certified eigenpairs/s)")
-    ap.add_argument("--serving", type=int, default=0,
-                    help="also run N serving requests (Si8..Si64 mix, Poisson arrivals) and report their p50/p99")
+    ap.add_argument("--serving", type=int, default=60,
+                    help="serving leg: N requests (Si8..Si64 mix, Poisson arrivals at 0.7 x saturation) through the "
+                         "serving engine, p50/p99 of arrival -> certified result (0: skip)")
     ap.add_argument("--serving-workers", type=int, default=4)
+    ap.add_argument("--serving-budget-gib", type=float, default=16.0, help="HBM admission budget of the serving leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 8 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     ap.add_argument("--real-space-factor", type=float, default=1.3,
@@ -302,8 +304,12 @@ def main():
         from sdft_amd import serving_bench
         op.close()
         out["serving"] = serving_bench.serving(args.serving, workers=args.serving_workers,
+                                               budget_gib=args.serving_budget_gib,
                                                davidson=dict(search_precision=1, subspace_factor=args.subspace_factor,
-                                                             stagnation_window=args.stagnation_window))
+                                                             stagnation_window=args.stagnation_window,
+                                                             refinement=args.refinement,
+                                                             inner_iterations=args.inner_iterations,
+                                                             search_handoff=args.search_handoff))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prob, fp64[-1]["iterations"])
     print(json.dumps(out))

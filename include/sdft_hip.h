@@ -313,6 +313,11 @@ int sdft_hip_tridiagonal_eigen(sdft_hip_context* ctx, size_t n, const double* d,
 int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* params, const double* initial,
                       sdft_hip_davidson_result* result);
 
+/* Device memory held by an operator: its arrays, work buffers and the
+ * Davidson buffers it keeps for its next solve (no counterpart in the
+ * reference; the serving loop admits requests against an HBM budget). */
+int sdft_hip_operator_device_bytes(sdft_hip_operator* op, size_t* bytes);
+
 /* Batched k-point application (no counterpart as one call in the reference,
  * whose k-points are independent PlaneWaveOperator applications on OpenMP
  * teams, src/scf_core.cpp:3051-3061): H psi and S psi of `count` device

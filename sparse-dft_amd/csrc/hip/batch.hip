@@ -311,4 +311,11 @@ hipError_t launch_transpose_projectors(const void* in, int np, int n, bool fp64,
   return hipGetLastError();
 }
 
+
+// Loads this file's code object on the calling thread (see module_prime.hpp).
+hipError_t prime_module_batch() {
+  hipFuncAttributes a{};
+  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(k_transpose_projectors));
+}
+
 }  // namespace sdft_hip

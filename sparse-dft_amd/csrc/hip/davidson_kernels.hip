@@ -573,4 +573,11 @@ hipError_t launch_zero_lower(double* a, long lda, long m, hipStream_t s) {
   SDFT_CHECKED(hipLaunchKernelGGL(k_zero_lower, dim3(square_blocks(m)), dim3(256), 0, s, a, lda, m));
 }
 
+
+// Loads this file's code object on the calling thread (see module_prime.hpp).
+hipError_t prime_module_davidson_kernels() {
+  hipFuncAttributes a{};
+  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(k_real_dots));
+}
+
 }  // namespace sdft_hip

@@ -540,4 +540,11 @@ hipError_t launch_zaxpby(int M, int N, double alpha, const double* X, long ldx, 
   return hipGetLastError();
 }
 
+
+// Loads this file's code object on the calling thread (see module_prime.hpp).
+hipError_t prime_module_gemm() {
+  hipFuncAttributes a{};
+  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(k_splitk_reduce));
+}
+
 }  // namespace sdft_hip

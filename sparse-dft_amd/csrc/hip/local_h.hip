@@ -501,4 +501,11 @@ hipError_t launch_potential_to_natural(const GridPlanDev& p, const double* v64, 
   return hipGetLastError();
 }
 
+
+// Loads this file's code object on the calling thread (see module_prime.hpp).
+hipError_t prime_module_local_h() {
+  hipFuncAttributes a{};
+  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(k_sticks_backward<double>));
+}
+
 }  // namespace sdft_hip

@@ -206,4 +206,11 @@ template hipError_t launch_rs_images<double>(const RealSpaceDev&, void*, long, i
 template hipError_t launch_rs_images<float>(const RealSpaceDev&, void*, long, int, const float*, int, const double*,
                                             long, hipStream_t);
 
+
+// Loads this file's code object on the calling thread (see module_prime.hpp).
+hipError_t prime_module_realspace() {
+  hipFuncAttributes a{};
+  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(k_rs_gather<double>));
+}
+
 }  // namespace sdft_hip

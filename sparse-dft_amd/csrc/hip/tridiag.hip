@@ -370,7 +370,7 @@ bool hetrd_plan(int n, int* rows_per_group, int* groups) {
   if (lds_bytes(n, rpw) > budget) return false;
   const int g = (n + rpw - 1) / rpw;
   if (cus > 0 && g > cus) return false;
-  // every workgroup must be resident at once (cooperative launch)
+  // every workgroup must be resident at once (the grid barrier waits for all)
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_hetrd_lds), kThreads,
                                                    lds_bytes(n, rpw)) != hipSuccess ||
@@ -407,15 +407,27 @@ hipError_t launch_hetrd(const double* a, long lda, int n, double* refl, long ldr
   const Z* az = reinterpret_cast<const Z*>(a);
   Z* rz = reinterpret_cast<Z*>(refl);
   Z* tz = reinterpret_cast<Z*>(tau);
-  void* args[] = {(void*)&az, (void*)&lda, (void*)&n, (void*)&rpw, (void*)&rz, (void*)&ldr, (void*)&d,
-                  (void*)&e, (void*)&tz, (void*)&pub_p, (void*)&pub_c, (void*)&flags, (void*)&abort_flag};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_hetrd_lds), dim3(groups), dim3(kThreads), args,
-                                    static_cast<unsigned>(lds_bytes(n, rpw)), s);
+  // A plain launch: the plan keeps the grid within one workgroup per CU, so
+  // on an idle device every workgroup is resident; when other work (another
+  // stream or process) holds CUs, the bounded wait expires and the caller
+  // redoes the solve by divide and conquer. A cooperative launch would
+  // instead serialise the device for the reduction, and it is refused when
+  // several processes share the device.
+  hipLaunchKernelGGL(k_hetrd_lds, dim3(groups), dim3(kThreads), lds_bytes(n, rpw), s, az, lda, n, rpw, rz, ldr, d, e,
+                     tz, pub_p, pub_c, flags, abort_flag);
+  return hipGetLastError();
 }
 
 const int* hetrd_abort_flag(const void* workspace, int n) {
   const Z* pub = reinterpret_cast<const Z*>(workspace);
   return reinterpret_cast<const int*>(pub + 4 * size_t(n));
+}
+
+
+// Loads this file's code object on the calling thread (see module_prime.hpp).
+hipError_t prime_module_tridiag() {
+  hipFuncAttributes a{};
+  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(k_hetrd_lds));
 }
 
 }  // namespace sdft_hip

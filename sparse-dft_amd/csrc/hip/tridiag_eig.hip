@@ -338,4 +338,11 @@ hipError_t launch_tridiag_vectors(int n, const double* d, const double* e, const
   return hipGetLastError();
 }
 
+
+// Loads this file's code object on the calling thread (see module_prime.hpp).
+hipError_t prime_module_tridiag_eig() {
+  hipFuncAttributes a{};
+  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(k_tridiag_values));
+}
+
 }  // namespace sdft_hip

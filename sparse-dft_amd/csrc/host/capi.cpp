@@ -593,6 +593,13 @@ int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* par
   });
 }
 
+int sdft_hip_operator_device_bytes(sdft_hip_operator* op, size_t* bytes) {
+  return guarded([&] {
+    if (!op || !bytes) throw std::invalid_argument("HIP: invalid device-bytes arguments");
+    *bytes = op->impl.device_bytes() + sdft_hip::davidson_cache_bytes(op->impl);
+  });
+}
+
 int sdft_hip_batch_create(sdft_hip_context* ctx, size_t count, sdft_hip_operator* const* ops, size_t columns,
                           int precision, sdft_hip_batch** out) {
   return guarded([&] {

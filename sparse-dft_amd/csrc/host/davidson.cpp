@@ -30,6 +30,16 @@ struct Workspace {
   bool busy = false;
 };
 
+std::size_t workspace_bytes(const Workspace& w) {
+  std::size_t b = 0;
+  for (const DevBuf<double>* d : {&w.basis, &w.hb, &w.sb, &w.ch, &w.cs, &w.x, &w.sx, &w.hx, &w.res, &w.small, &w.w1,
+                                  &w.w2, &w.w3, &w.w4, &w.m1, &w.m2, &w.m3, &w.rr, &w.rrb, &w.qbuf, &w.corr,
+                                  &w.retained})
+    b += d->bytes();
+  if (w.eig) b += w.eig->device_bytes();
+  return b;
+}
+
 std::shared_ptr<Workspace> acquire_workspace(DeviceOperator& op) {
   std::shared_ptr<void>& slot = op.davidson_cache();
   auto ws = std::static_pointer_cast<Workspace>(slot);
@@ -1109,6 +1119,11 @@ DavidsonOutput Solver::run(const C* initial) {
   return out;
 }
 }  // namespace
+
+std::size_t davidson_cache_bytes(DeviceOperator& op) {
+  auto ws = std::static_pointer_cast<Workspace>(op.davidson_cache());
+  return ws ? workspace_bytes(*ws) : 0;
+}
 
 std::vector<long> davidson_order(const DeviceOperator& op) {
   const std::vector<double>& hd = op.h_diagonal();

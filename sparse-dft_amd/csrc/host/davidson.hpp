@@ -82,6 +82,8 @@ struct StrictStepOutput {
 StrictStepOutput strict_step(DeviceOperator& op, int kind, long roots, double tolerance, const std::complex<double>* x,
                              const std::complex<double>* direction);
 
+// Device bytes of the Davidson buffers the operator keeps for its next solve.
+std::size_t davidson_cache_bytes(DeviceOperator& op);
 // Canonical order of the basis by H/S diagonal (src/pw_core.cpp:1016-1021).
 std::vector<long> davidson_order(const DeviceOperator& op);
 // The Davidson seed block of `roots` roots at tolerance `tol` (warm start

@@ -26,6 +26,7 @@ class HermitianEigensolver {
   // (CUs held by other work past the wait bound) and that were redone by
   // rocSOLVER zheevd on the unchanged input
   long barrier_fallbacks() const { return barrier_fallbacks_; }
+  std::size_t device_bytes() const { return refl_.bytes() + z_.bytes() + small_.bytes() + orth_.bytes() + work_.bytes(); }
 
  private:
   Context& ctx_;

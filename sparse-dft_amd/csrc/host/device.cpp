@@ -9,11 +9,13 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <tuple>
 
 #include "../hip/slot_order.hpp"
 #include "../hip/fft_lengths.hpp"
+#include "../hip/module_prime.hpp"
 #include "snapshot.hpp"
 
 namespace sdft_hip {
@@ -157,8 +159,30 @@ void DeviceGrid::upload(const HostGridPlan& h, hipStream_t s) {
 }
 
 // ------------------------------------------------------------ Context
+namespace {
+// The kernel code objects of this library and rocBLAS's GEMM kernel library
+// are loaded once per process, on the thread that creates the first context,
+// rather than lazily on whichever thread first launches a kernel: a solve
+// service's worker threads end before the process does, and a process that
+// first used the libraries on such a thread ended in a segmentation fault at
+// exit. Per device (each device loads its own code objects).
+void prime_libraries(int device) {
+  static std::mutex m;
+  static std::vector<int> primed;
+  std::lock_guard<std::mutex> lock(m);
+  if (std::find(primed.begin(), primed.end(), device) != primed.end()) return;
+  rocblas_initialize();
+  for (hipError_t (*f)() : {prime_module_local_h, prime_module_gemm, prime_module_davidson_kernels,
+                            prime_module_tridiag, prime_module_tridiag_eig, prime_module_realspace,
+                            prime_module_batch})
+    check(f(), "kernel code object");
+  primed.push_back(device);
+}
+}  // namespace
+
 Context::Context(int device) : device_(device) {
   check(hipSetDevice(device), "hipSetDevice");
+  prime_libraries(device);
   hipDeviceProp_t props{};
   check(hipGetDeviceProperties(&props, device), "hipGetDeviceProperties");
   name_ = props.name;
@@ -667,6 +691,18 @@ void* DeviceOperator::stick_buffer(int which, std::size_t columns, Precision pre
   if (columns > chunk_) throw std::invalid_argument("HIP: column block exceeds the stick-buffer chunk");
   const std::size_t bytes = columns * std::size_t(grid_.dev.nsticks) * shape_[2] * (prec == Precision::FP64 ? 16 : 8);
   return which ? static_cast<void*>(t2_.reserve(bytes)) : static_cast<void*>(t1_.reserve(bytes));
+}
+
+std::size_t DeviceOperator::device_bytes() const {
+  std::size_t b = grid_.stick_g.bytes() + grid_.stick_xy.bytes();
+  for (int a = 0; a < 3; ++a) b += grid_.tw64[a].bytes() + grid_.tw32[a].bytes();
+  b += onsite_d_.bytes() + onsite_q_.bytes() + v64_.bytes() + kinetic_d_.bytes() + b64_.bytes() + dblk_.bytes() +
+       qblk_.bytes() + v32_.bytes() + b32_.bytes() + atom_offset_.bytes() + atom_rows_.bytes() + block_offset_.bytes() +
+       t1_.bytes() + t2_.bytes() + p_.bytes() + w_.bytes();
+  b += rs_w64_.bytes() + rs_sdiag_.bytes() + rs_w32_.bytes() + rs_woff_.bytes() + rs_poff_.bytes() +
+       rs_points_.bytes() + rs_upoints_.bytes() + rs_estart_.bytes() + rs_eatom_.bytes() + rs_elocal_.bytes() +
+       rs_aoff_.bytes() + rs_planes_.bytes() + rs_psi_.bytes();
+  return b;
 }
 
 void DeviceOperator::apply(int which, const double* x, long columns, double* hy, double* sy, Precision prec) {

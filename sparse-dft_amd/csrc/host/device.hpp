@@ -60,6 +60,7 @@ struct DevBuf {
     p = nullptr;
     n = 0;
   }
+  std::size_t bytes() const { return n * sizeof(T); }
   T* reserve(std::size_t count) {
     if (count > n) {
       release();
@@ -282,6 +283,9 @@ class DeviceOperator {
   // device buffers of the Davidson solver, kept between solves of this
   // operator (type known to davidson.cpp only)
   std::shared_ptr<void>& davidson_cache() { return davidson_cache_; }
+  // device bytes held by the operator's own arrays and work buffers (the
+  // Davidson buffers kept for the next solve are counted by the solver)
+  std::size_t device_bytes() const;
   std::size_t max_chunk() const { return chunk_; }
   const GridPlanDev& grid_plan() const { return grid_.dev; }
   // device arrays of the nonlocal part (atom-block form 3): projectors

@@ -318,10 +318,33 @@ class Context:
         return w
 
 
-class Operator:
-    """Device-resident plane-wave operator built from a synthetic Problem."""
+def pack_problem(prob):
+    """The operator description of a Problem as contiguous arrays (the layout
+    of sdft_hip_operator_desc): Miller indices, kinetic energies (eV), the
+    Fourier potential, projector rows per atom, row-major projectors and
+    onsite blocks, plus the regularisation and grid shape."""
+    return dict(
+        hkl=np.ascontiguousarray(prob.hkl, dtype=np.int32),
+        kin=np.ascontiguousarray(prob.kinetic_ev, dtype=np.float64),
+        vh=np.ascontiguousarray(prob.v_hkl, dtype=np.int32).reshape(-1, 3),
+        vv=np.ascontiguousarray(np.stack([np.real(prob.v_values), np.imag(prob.v_values)], axis=1)),
+        rows=np.ascontiguousarray(prob.atom_rows, dtype=np.int64),
+        amp=np.ascontiguousarray(np.concatenate([np.ravel(a) for a in prob.amplitudes]) if len(prob.amplitudes)
+                                 else np.zeros(0, dtype=np.complex128)),
+        d=np.ascontiguousarray(np.concatenate([np.ravel(a) for a in prob.d_blocks]) if len(prob.d_blocks)
+                               else np.zeros(0, dtype=np.complex128)),
+        q=np.ascontiguousarray(np.concatenate([np.ravel(a) for a in prob.q_blocks]) if len(prob.q_blocks)
+                               else np.zeros(0, dtype=np.complex128)),
+        regularization=float(prob.regularization),
+        shape=tuple(int(v) for v in prob.shape),
+    )
 
-    def __init__(self, ctx: Context, prob, shape=None, _handle=None):
+
+class Operator:
+    """Device-resident plane-wave operator built from a synthetic Problem (or
+    its packed description, pack_problem)."""
+
+    def __init__(self, ctx: Context, prob, shape=None, _handle=None, packed=None):
         self.ctx = ctx
         if _handle is not None:
             self.handle = _handle
@@ -329,20 +352,11 @@ class Operator:
             self.n = None
             self._describe()
             return
-        self.n = prob.n
-        self._keep = dict(
-            hkl=np.ascontiguousarray(prob.hkl, dtype=np.int32),
-            kin=np.ascontiguousarray(prob.kinetic_ev, dtype=np.float64),
-            vh=np.ascontiguousarray(prob.v_hkl, dtype=np.int32),
-            vv=np.ascontiguousarray(np.stack([prob.v_values.real, prob.v_values.imag], axis=1)),
-            rows=np.ascontiguousarray(prob.atom_rows, dtype=np.int64),
-            amp=np.ascontiguousarray(np.concatenate([a.ravel() for a in prob.amplitudes])),
-            d=np.ascontiguousarray(np.concatenate([a.ravel() for a in prob.d_blocks])),
-            q=np.ascontiguousarray(np.concatenate([a.ravel() for a in prob.q_blocks])),
-        )
+        self._keep = packed if packed is not None else pack_problem(prob)
         k = self._keep
+        self.n = int(k["hkl"].shape[0])
         desc = OperatorDesc()
-        desc.n = prob.n
+        desc.n = self.n
         desc.hkl = _p(k["hkl"], _ip)
         desc.kinetic_ev = _p(k["kin"])
         desc.ev_per_hartree = 27.211386245988
@@ -354,8 +368,8 @@ class Operator:
         desc.amplitudes = _p(k["amp"])
         desc.d = _p(k["d"])
         desc.q = _p(k["q"])
-        desc.regularization = prob.regularization
-        shp = shape if shape is not None else prob.shape
+        desc.regularization = k["regularization"]
+        shp = shape if shape is not None else k["shape"]
         for a in range(3):
             desc.shape[a] = int(shp[a])
         self.handle = ct.c_void_p()
@@ -462,6 +476,12 @@ class Operator:
         _check(lib().sdft_hip_davidson_seeds(self.handle, ct.c_size_t(roots), ct.c_double(tolerance),
                                              _p(init) if init is not None else None, _p(block), ct.byref(cols)))
         return block[:, :cols.value]
+
+    def device_bytes(self):
+        """Device memory held by the operator (arrays, work buffers, kept Davidson buffers)."""
+        v = ct.c_size_t()
+        _check(lib().sdft_hip_operator_device_bytes(self.handle, ct.byref(v)))
+        return v.value
 
     def strict_step(self, kind, x, direction=None, tolerance=1e-12):
         """One strict certificate-path step on the block x (n x roots) from its

@@ -42,12 +42,15 @@ def request_pool(pool, seed=4):
     return out
 
 
-def run(svc, pool, requests, rate=None, seed=5, davidson=None):
+def run(svc, pool, requests, rate=None, seed=5, davidson=None, faults=None):
     """Submit `requests` solves (cycling through `pool`); rate None = all at
-    t = 0 (closed loop), else Poisson arrivals at `rate` per second.
-    Returns per-request records."""
+    t = 0 (closed loop), else Poisson arrivals at `rate` per second. faults:
+    {request index: fault kind} injected into single requests (their failure
+    is counted, and they are left out of the latency records). Returns
+    (records of the certified requests, wall time, failures)."""
     rng = np.random.default_rng(seed)
     davidson = dict(davidson or {})
+    faults = dict(faults or {})
     t0 = time.perf_counter()
     arrivals = np.zeros(requests) if rate is None else np.cumsum(rng.exponential(1.0 / rate, requests))
     futs = []
@@ -57,16 +60,28 @@ def run(svc, pool, requests, rate=None, seed=5, davidson=None):
         if delay > 0:
             time.sleep(delay)
         job = pool[i % len(pool)]
-        futs.append((job, svc.submit(job["prob"], job["roots"], job["init"], arrival=target, **davidson)))
-    records = []
-    for job, f in futs:
-        r = f.result()
+        kw = dict(davidson)
+        if i in faults:
+            kw["fault"] = faults[i]
+        futs.append((i, job, svc.submit(job["prob"], job["roots"], job["init"], arrival=target, **kw)))
+    records, failures = [], []
+    for i, job, f in futs:
+        try:
+            r = f.result()
+        except Exception as e:  # noqa: BLE001 - an injected fault fails its request alone
+            if i not in faults:
+                raise
+            failures.append(dict(request=i, fault=faults[i], error=type(e).__name__))
+            continue
+        if i in faults:
+            raise RuntimeError(f"request {i} with an injected {faults[i]} fault did not fail")
         if not r["converged"]:
             raise RuntimeError("a serving request did not certify")
         records.append(dict(atoms=job["atoms"], ecut=job["ecut"], roots=job["roots"], latency=r["finish"] - r["arrival"],
-                            service=r["finish"] - r["start"], finish=r["finish"], arrival=r["arrival"],
-                            max_residual=float(r["residual_norms"].max())))
-    return records, time.perf_counter() - t0
+                            queue=r["admitted"] - r["arrival"], service=r["finish"] - r["start"], finish=r["finish"],
+                            arrival=r["arrival"], max_residual=float(r["residual_norms"].max()),
+                            device_bytes=r["device_bytes"], estimated_bytes=r["estimated_bytes"]))
+    return records, time.perf_counter() - t0, failures
 
 
 def summary(records, wall):
@@ -74,22 +89,32 @@ def summary(records, wall):
     pairs = sum(r["roots"] for r in records)
     return dict(requests=len(records), p50_latency_s=float(np.percentile(lat, 50)),
                 p99_latency_s=float(np.percentile(lat, 99)), mean_latency_s=float(lat.mean()),
+                mean_queue_s=float(np.mean([r["queue"] for r in records])),
                 throughput_requests_per_s=len(records) / wall, certified_eigenpairs_per_s=pairs / wall,
-                max_certified_residual_ha=max(r["max_residual"] for r in records))
+                max_certified_residual_ha=max(r["max_residual"] for r in records),
+                max_device_bytes_over_estimate=max(r["device_bytes"] / r["estimated_bytes"] for r in records))
 
 
-def serving(requests=200, pool=20, workers=4, load=0.7, davidson=None):
+def serving(requests=60, pool=16, workers=4, load=0.7, budget_gib=16.0, davidson=None):
+    """Closed-loop saturation pass, then `requests` Poisson arrivals at `load`
+    x saturation through the serving engine (HBM budget `budget_gib`), with
+    one injected nonfinite-application fault and one injected over-budget
+    request in the stream; p50/p99 of arrival -> certified result."""
     jobs = request_pool(pool)
-    svc = serve.SolveService(workers=workers)
+    budget = int(budget_gib * (1 << 30))
+    svc = serve.ServingEngine(workers=workers, hbm_budget=budget)
     try:
         run(svc, jobs[:len(CELLS)], len(CELLS), davidson=davidson)  # warm-up: one of each size
-        closed, wall_closed = run(svc, jobs, max(len(jobs), requests // 4), davidson=davidson)
+        closed, wall_closed, _ = run(svc, jobs, len(jobs), davidson=davidson)
         saturation = len(closed) / wall_closed
         rate = load * saturation
-        opened, wall_open = run(svc, jobs, requests, rate=rate, davidson=davidson)
+        faults = {requests // 3: "nonfinite", (2 * requests) // 3: "budget"}
+        opened, wall_open, failures = run(svc, jobs, requests, rate=rate, davidson=davidson, faults=faults)
     finally:
         svc.close()
     out = summary(opened, wall_open)
     out.update(saturation_requests_per_s=saturation, arrival_rate_per_s=rate, load=load, workers=workers,
-               pool=pool, mix="Si8/16/32/64 uniform, Ecut in {10..15} Ha, 2.5 bands/atom")
+               pool=pool, hbm_budget_gib=budget_gib, max_in_flight_gib=svc.max_in_flight_bytes / (1 << 30),
+               max_concurrent=svc.max_concurrent, injected_failures=failures,
+               mix="Si8/16/32/64 uniform, Ecut in {10..15} Ha, 2.5 bands/atom")
     return out

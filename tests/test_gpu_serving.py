@@ -26,7 +26,9 @@ def _jobs():
 
 
 @pytest.mark.skip(reason="run alone in its own process this test passed and the interpreter then ended with a "
-                         "segmentation fault at exit; the cause is not identified, so it stays off the GPU")
+                         "segmentation fault at exit; teardown hazards were removed (reference-counted contexts, "
+                         "ordered close at exit) but the cause was not shown, so this case stays off the GPU; "
+                         "test_serving_engine_mix covers the alone-versus-mixed equality")
 def test_mixed_requests_match_solo_solves():
     jobs = _jobs()
     svc = serve.SolveService(workers=3, davidson=dict(search_precision=1))
@@ -42,6 +44,46 @@ def test_mixed_requests_match_solo_solves():
         try:
             op = native.Operator(ctx, prob)
             alone = op.davidson(roots, 256, 1e-12, init, search_precision=1)
+            op.close()
+        finally:
+            ctx.close()
+        assert np.array_equal(alone["eigenvalues"], r["eigenvalues"])
+        assert np.array_equal(alone["eigenvectors"], r["eigenvectors"])
+
+
+DAVIDSON = dict(search_precision=1, refinement=1, search_handoff=3e-3)
+
+
+def test_serving_engine_mix():
+    """Config-4 serving: Si8/Si16 requests at mixed cutoffs submitted at once
+    to the serving engine (worker processes, one HIP context each) with an HBM
+    budget that admits only part of them at a time. Every certified result equals its solo solve bitwise; the
+    request with an injected nonfinite application and the one with an
+    over-budget estimate fail alone; the in-flight estimate never exceeds
+    the budget while several requests ran concurrently."""
+    jobs = _jobs()
+    sizes = sorted(serve.estimate_request_bytes(p, roots, 8) for p, roots, _ in jobs)
+    budget = sizes[-1] + sizes[-2]
+    eng = serve.ServingEngine(workers=3, hbm_budget=budget, davidson=DAVIDSON)
+    try:
+        futs = [eng.submit(p, roots, init) for p, roots, init in jobs]
+        bad_nan = eng.submit(jobs[0][0], jobs[0][1], jobs[0][2], fault="nonfinite")
+        bad_budget = eng.submit(jobs[1][0], jobs[1][1], jobs[1][2], fault="budget")
+        mixed = [f.result(timeout=300) for f in futs]
+        with pytest.raises(native.HipError, match="nonfinite"):
+            bad_nan.result(timeout=300)
+        with pytest.raises(serve.RequestRejected):
+            bad_budget.result(timeout=300)
+    finally:
+        eng.close()
+    assert eng.max_in_flight_bytes <= budget and eng.max_concurrent > 1
+    for (prob, roots, init), r in zip(jobs, mixed):
+        assert r["converged"] and r["residual_norms"].max() <= 1e-12
+        assert r["device_bytes"] <= r["estimated_bytes"]
+        ctx = native.Context(0)
+        try:
+            op = native.Operator(ctx, prob)
+            alone = op.davidson(roots, 256, 1e-12, init, **DAVIDSON)
             op.close()
         finally:
             ctx.close()
