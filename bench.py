@@ -32,35 +32,117 @@ def algorithmic_local_bytes(npw, count, m, c, r):
     return m * (2 * npw * c + 4 * count * c) + count * r + npw * (4 + r)
 
 
-def cpu_baseline(prob, iterations_fp64):
-    """Oracle (FP64 C++/OpenMP restatement of the reference CPU path, GEMMs and
-    dense eigensolves through CBLAS/LAPACKE as the reference's CPU build does)
-    on a bounded sample: Davidson truncated after 1 and 2 Rayleigh-Ritz steps;
-    the per-iteration cost extrapolates a full solve of the FP64 path's
-    iteration count."""
+def host_cpu():
+    """CPUs this process may run on, their physical cores and the CPU model
+    (from the affinity mask, /sys topology and /proc/cpuinfo)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        try:
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/core_id") as f:
+                core = f.read().strip()
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/physical_package_id") as f:
+                pkg = f.read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add((None, c))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(cpus=len(cpus), physical_cores=len(cores), model=model)
+
+
+def cpu_baseline(prob):
+    """Oracle (FP64 C++/OpenMP restatement of the reference CPU path; GEMMs
+    and dense eigensolves through CBLAS/LAPACKE as the reference's CPU build;
+    FFTs its own mixed radix, FFTW being absent) on this host, threads bound
+    close: one complete certified Si64 solve (the headline workload) and one
+    Si8 solve, timed whole; the per-column H psi / S psi time of Si64 on one
+    thread and on all, and of the two-species slab on all."""
+    cpu = host_cpu()
+    threads = min(cpu["cpus"], int(os.environ.get("OMP_NUM_THREADS", cpu["cpus"])))
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle
     from sdft_amd import synthetic as syn
     blas = pyoracle.numpy_openblas_path()
     if blas:
         pyoracle.bind_blas(blas)
+    pyoracle.set_threads(threads)
     op = pyoracle.Operator(prob)
     init = syn.initial_psi(prob, ROOTS, 1)
     t0 = time.perf_counter()
-    op.davidson(ROOTS, 1, 1e-12, init)
-    t1 = time.perf_counter() - t0
+    r = op.davidson(ROOTS, 256, 1e-12, init, subspace_factor=8)
+    t_si64 = time.perf_counter() - t0
+    if not r["converged"]:
+        raise RuntimeError("the CPU baseline solve did not certify")
+    x = syn.initial_psi(prob, 16, 1, extra=9)
     t0 = time.perf_counter()
-    op.davidson(ROOTS, 2, 1e-12, init)
-    t2 = time.perf_counter() - t0
-    per_iter = max(t2 - t1, 1e-9)
-    total = t1 + per_iter * max(iterations_fp64 - 1, 0)
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": ROOTS / total, "unit": "certified eigenpairs/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle FP64 Davidson (in-repo mixed-radix FFT, {'OpenBLAS zgemm/zhegvd' if blas else 'plain loops'}) "
-                       f"on the same Si64 problem truncated after 1 and 2 Rayleigh-Ritz steps "
-                       f"({t1:.1f} s, {t2:.1f} s); full solve extrapolated to the FP64 path's {iterations_fp64} "
-                       f"iterations = {total:.0f} s"),
-            "seconds_sample": t1 + t2}
+    op.apply(x, "pair")
+    col_all = (time.perf_counter() - t0) / 16
+    pyoracle.set_threads(1)
+    t0 = time.perf_counter()
+    op.apply(x[:, :2], "pair")
+    col_one = (time.perf_counter() - t0) / 2
+    pyoracle.set_threads(threads)
+    p0 = syn.config_problem(0)
+    op0 = pyoracle.Operator(p0)
+    t0 = time.perf_counter()
+    r0 = op0.davidson(16, 256, 1e-12, syn.initial_psi(p0, 16, 0))
+    t_si8 = time.perf_counter() - t0
+    p3 = syn.config_problem(3)
+    op3 = pyoracle.Operator(p3)
+    x3 = syn.initial_psi(p3, 16, 3)
+    t0 = time.perf_counter()
+    op3.apply(x3, "pair")
+    col_slab = (time.perf_counter() - t0) / 16
+    return {"value": ROOTS / t_si64, "unit": "certified eigenpairs/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle FP64 Davidson (reference structure, subspace factor 8, in-repo mixed-radix FFT, "
+                       f"{'OpenBLAS zgemm/zhegvd' if blas else 'plain loops'}), one complete certified Si64 solve "
+                       f"of the same problem and warm block: {t_si64:.1f} s, {r['iterations']} iterations"),
+            "seconds_sample": t_si64, "iterations": r["iterations"],
+            "host": dict(cpu, threads=threads, omp_proc_bind=os.environ.get("OMP_PROC_BIND")),
+            "si8_certified_eigenpairs_per_s": 16 / t_si8, "si8_iterations": r0["iterations"],
+            "si64_pair_apply_ms_per_column": {"threads": 1e3 * col_all, "one_thread": 1e3 * col_one},
+            "slab_pair_apply_ms_per_column": 1e3 * col_slab}
+
+
+def kpoint_batch_leg(ctx):
+    """Config 2: the Si8 4x4x4 Monkhorst-Pack mesh (64 k-points, 1622-1653
+    plane waves, one 32^3 grid, 32 bands each) applied as one graph replay of
+    the fused persistent kernel, FP64 and FP32, against the 64 per-k-point
+    device applications. Work per launch: the FFT count 2 x 5 N log2 N per
+    column (both transforms) and the HBM model of the local-H roofline
+    B_loc = m (2 npw c + 4 N c) + N r + npw (4 + r) per k-point block."""
+    from sdft_amd import native
+    from sdft_amd import synthetic as syn
+    ks = syn.mp_kpoints(4)
+    probs = [syn.config_problem(2, kfrac=tuple(k)) for k in ks]
+    ops = [native.Operator(ctx, p) for p in probs]
+    cols = 32
+    count = 32 ** 3
+    out = {"kpoints": len(ops), "columns_per_kpoint": cols, "npw": [min(p.n for p in probs), max(p.n for p in probs)]}
+    flops = 2 * 5 * count * np.log2(count) * cols * len(ops)
+    try:
+        for prec, name, c, r, peak in ((0, "fp64", 16, 8, 78.6), (1, "fp32", 8, 4, 157.3)):
+            b = native.Batch(ctx, ops, cols, precision=prec)
+            ms = b.benchmark(20)
+            seq = sum(op.benchmark(cols, "pair", precision=prec, repetitions=3)["apply_ms"] for op in ops)
+            model = sum(algorithmic_local_bytes(p.n, count, cols, c, r) for p in probs)
+            out[name] = {"replay_ms": ms, "plan": b.plan(), "per_k_applies_ms": seq, "speedup_vs_per_k": seq / ms,
+                         "fft_tflops": flops / (ms * 1e-3) / 1e12, "fft_frac_of_vector_peak": flops / (ms * 1e-3) / 1e12 / peak,
+                         "local_model_gbs": model / (ms * 1e-3) / 1e9, "local_model_frac_of_hbm": model / (ms * 1e-3) / 1e9
+                         / PEAK_HBM_GBS}
+            b.close()
+    finally:
+        for op in ops:
+            op.close()
+    return out
 
 
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "si64_hbm_traffic.md")
@@ -134,11 +216,14 @@ def main():
                          "serving engine, p50/p99 of arrival -> certified result (0: skip)")
     ap.add_argument("--serving-workers", type=int, default=4)
     ap.add_argument("--serving-budget-gib", type=float, default=16.0, help="HBM admission budget of the serving leg")
+    ap.add_argument("--kpoint-batch", type=int, default=1,
+                    help="also time the batched Si8 4x4x4 k-point application (one graph replay, 64 k x 32 bands)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 8 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     ap.add_argument("--real-space-factor", type=float, default=1.3,
                     help="also time a solve with the real-space projector search (0: skip)")
     args = ap.parse_args()
+    os.environ.setdefault("OMP_PROC_BIND", "close")  # before the oracle's OpenMP runtime loads
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -241,6 +326,20 @@ def main():
                 "local_fp64_160": {"ms": bench64["local_ms"], "GB/s": algorithmic_local_bytes(npw, count, ROOTS, 16, 8)
                                    / (bench64["local_ms"] * 1e-3) / 1e9}}
 
+    # "block size 32": the bounded band-group solver with groups of 32 roots
+    # (the reference's SDFT_BAND_GROUP_SOLVER=1, SDFT_BAND_GROUP_SIZE=32), FP64
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    rbg = op.davidson(ROOTS, 256, 1e-12, init, band_group_size=32, subspace_factor=args.subspace_factor)
+    ctx.synchronize()
+    t_bg = time.perf_counter() - t0
+    band_groups = {"group_size": 32, "value": ROOTS / t_bg, "latency_s": t_bg, "converged": rbg["converged"],
+                   "iterations": rbg["iterations"],
+                   "max_eigenvalue_error_vs_fp64_ha": float(np.abs(rbg["eigenvalues"] - fp64[-1]["eigenvalues"]).max()),
+                   "max_certified_residual_ha": float(rbg["residual_norms"].max())}
+
+    kpoint_batch = kpoint_batch_leg(ctx) if args.kpoint_batch else None
+
     real_space = None
     if args.real_space_factor > 0:
         # Real-space projector search as the search operator (the certificate
@@ -294,6 +393,9 @@ def main():
         "kernels": kernels,
         "isolated_local_h": isolated,
     }
+    out["band_groups_32"] = band_groups
+    if kpoint_batch is not None:
+        out["kpoint_batch"] = kpoint_batch
     if real_space is not None:
         out["real_space_search"] = real_space
     if args.timing:
@@ -311,7 +413,7 @@ def main():
                                                              inner_iterations=args.inner_iterations,
                                                              search_handoff=args.search_handoff))
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(prob, fp64[-1]["iterations"])
+        out["cpu_baseline"] = cpu_baseline(prob)
     print(json.dumps(out))
 
 

@@ -127,6 +127,8 @@ DavidsonResult davidson(const Operator& op, std::size_t roots, std::size_t max_i
 // Binds CBLAS zgemm / LAPACKE zhegvd, zheevd (ILP64, numpy's OpenBLAS naming)
 // from the shared library at `path`; an empty path restores the plain loops.
 bool bind_blas(const char* path);
+// OpenMP threads of the oracle loops and of the bound BLAS
+void set_threads(int n);
 bool lapack_hegvd(std::size_t n, C* a, C* b, double* w);  // false when unbound
 bool lapack_heevd(std::size_t n, C* a, double* w);
 

@@ -67,6 +67,10 @@ int orc_bind_blas(const char* path) {
   });
 }
 
+int orc_set_threads(int n) {
+  return guarded([&] { orc::set_threads(n); });
+}
+
 // In-place unnormalised 3-D FFT of a row-major grid (sign -1 forward, +1 backward).
 int orc_fft3d(const int* shape, double* grid, int sign) {
   return guarded([&] {

@@ -174,8 +174,20 @@ zhegvd64_t g_zhegvd = nullptr;
 zheevd64_t g_zheevd = nullptr;
 }  // namespace
 
+namespace {
+using set_threads_t = void (*)(int);
+set_threads_t g_blas_threads = nullptr;
+}  // namespace
+
+void set_threads(int n) {
+  if (n < 1) throw std::invalid_argument("thread count must be positive");
+  omp_set_num_threads(n);
+  if (g_blas_threads) g_blas_threads(n);
+}
+
 bool bind_blas(const char* path) {
   if (!path || !*path) {
+    g_blas_threads = nullptr;
     g_zgemm = nullptr;
     g_zhegvd = nullptr;
     g_zheevd = nullptr;
@@ -186,6 +198,7 @@ bool bind_blas(const char* path) {
   g_zgemm = reinterpret_cast<zgemm64_t>(dlsym(h, "scipy_cblas_zgemm64_"));
   g_zhegvd = reinterpret_cast<zhegvd64_t>(dlsym(h, "scipy_LAPACKE_zhegvd64_"));
   g_zheevd = reinterpret_cast<zheevd64_t>(dlsym(h, "scipy_LAPACKE_zheevd64_"));
+  g_blas_threads = reinterpret_cast<set_threads_t>(dlsym(h, "scipy_openblas_set_num_threads64_"));
   return g_zgemm && g_zhegvd && g_zheevd;
 }
 

@@ -75,6 +75,11 @@ def bind_blas(path):
     _check(lib().orc_bind_blas(None if path is None else path.encode()))
 
 
+def set_threads(n):
+    """OpenMP threads of the oracle and of the bound BLAS."""
+    _check(lib().orc_set_threads(ct.c_int(int(n))))
+
+
 def fast_fft_length(m):
     return lib().orc_fast_fft_length(ct.c_int(m))
 
