@@ -155,6 +155,22 @@ int sdft_hip_operator_info(sdft_hip_operator* op, int shape[3], long* count, lon
  * device_pointers = 1: x, hy, sy are device pointers (n x columns). */
 int sdft_hip_operator_apply(sdft_hip_operator* op, int which, const double* x, size_t columns, double* hy, double* sy,
                             int precision, int device_pointers);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sdft_hip_operator_apply with device_pointers = 1 that
+ * allocates nothing. x, hy, sy are device pointers (n x columns); the stick
+ * buffers, projections and GEMM split-K partials are staged in a statically
+ * allocated SDFT_HIP_SCRATCH_BYTES array per GPU, held until the process
+ * ends. Columns go in passes of the most that fit; an operator one of whose
+ * columns needs more staging than the array holds is rejected
+ * (SDFT_HIP_OVERFLOW), and so is precision 1 on an operator whose real-space
+ * projector search is enabled. Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two contexts or threads at once) race.
+ * Returns with the outputs complete, like the allocating entry point, whose
+ * results it reproduces bit for bit when the block fits one pass. */
+#define SDFT_HIP_SCRATCH_BYTES (128u << 20)
+int sdft_hip_operator_apply_noalloc(sdft_hip_operator* op, int which, const double* x, size_t columns, double* hy,
+                                    double* sy, int precision);
 /* Averages `repetitions` device-resident applications of `columns` columns
  * with HIP events on the context stream (inputs resident in HBM).
  * kernel_ms[0..2]: local-H pipeline, projector GEMMs + onsite, whole apply. */

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../hip/davidson_kernels.hpp"
+#include "../hip/scratch.hpp"
 #include "../hip/tridiag_eig.hpp"
 #include "batch.hpp"
 #include "davidson.hpp"
@@ -482,6 +483,18 @@ int sdft_hip_operator_apply(sdft_hip_operator* op, int which, const double* x, s
     o.apply(which, dx.p, long(columns), dh.p, ds.p, prec);
     if (which != 1) check(hipMemcpyAsync(hy, dh.p, total * 8, hipMemcpyDeviceToHost, s), "copy h");
     if (which != 0) check(hipMemcpyAsync(sy, ds.p, total * 8, hipMemcpyDeviceToHost, s), "copy s");
+    o.context().sync();
+  });
+}
+
+int sdft_hip_operator_apply_noalloc(sdft_hip_operator* op, int which, const double* x, size_t columns, double* hy,
+                                    double* sy, int precision) {
+  return guarded([&] {
+    if (!op || !x || !columns || which < 0 || which > 2 || (which != 1 && !hy) || (which != 0 && !sy))
+      throw std::invalid_argument("HIP: invalid plane-wave block operator buffers");
+    static_assert(SDFT_HIP_SCRATCH_BYTES == sdft_hip::kScratchBytes, "scratch size of the C header");
+    auto& o = op->impl;
+    o.apply_noalloc(which, x, long(columns), hy, sy, to_precision(precision));
     o.context().sync();
   });
 }

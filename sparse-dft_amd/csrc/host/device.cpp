@@ -16,6 +16,7 @@
 #include "../hip/slot_order.hpp"
 #include "../hip/fft_lengths.hpp"
 #include "../hip/module_prime.hpp"
+#include "../hip/scratch.hpp"
 #include "snapshot.hpp"
 
 namespace sdft_hip {
@@ -183,6 +184,7 @@ void prime_libraries(int device) {
 Context::Context(int device) : device_(device) {
   check(hipSetDevice(device), "hipSetDevice");
   prime_libraries(device);
+  scratch_ = scratch_region(device);
   hipDeviceProp_t props{};
   check(hipGetDeviceProperties(&props, device), "hipGetDeviceProperties");
   name_ = props.name;
@@ -363,11 +365,20 @@ void Context::zgemm(Precision prec, int aop, int bop, long M, long N, long K, do
     throw std::overflow_error("HIP: GEMM dimension overflow");
   const std::size_t bytes = zgemm_workspace_bytes(int(M), int(N), int(K));
   double* w = bytes ? gemm_work(bytes) : nullptr;
+  zgemm_into(prec, aop, bop, M, N, K, alpha, A, lda, B, ldb, C, ldc, D, ldd, beta, w, bytes ? gemm_work_bytes() : 0);
+}
+
+void Context::zgemm_into(Precision prec, int aop, int bop, long M, long N, long K, double alpha, const double* A,
+                         long lda, const double* B, long ldb, double* C, long ldc, const double* D, long ldd,
+                         double beta, double* work, std::size_t work_bytes) {
+  if (M <= 0 || N <= 0) return;
+  if (std::max({M, N, K}) > long(std::numeric_limits<int>::max()))
+    throw std::overflow_error("HIP: GEMM dimension overflow");
   const LaunchEvents ev = K > 0 ? clock_launch(prec == Precision::FP64 ? kGemmFP64 : kGemmFP32, 8.0 * M * N * K,
                                                gemm_algorithmic_bytes(M, N, K, 16, beta != 0.0))
                                 : LaunchEvents{};
-  check(sdft_hip::zgemm(prec, aop, bop, int(M), int(N), int(K), alpha, A, lda, B, ldb, C, ldc, D, ldd, beta, w,
-                        bytes ? gemm_work_bytes() : 0, stream_, &ev),
+  check(sdft_hip::zgemm(prec, aop, bop, int(M), int(N), int(K), alpha, A, lda, B, ldb, C, ldc, D, ldd, beta, work,
+                        work_bytes, stream_, &ev),
         "zgemm launch");
 }
 
@@ -718,11 +729,100 @@ void DeviceOperator::apply(int which, const double* x, long columns, double* hy,
   for (long c0 = 0; c0 < columns; c0 += long(chunk_)) {
     const long c = std::min<long>(long(chunk_), columns - c0);
     const long off = 2 * c0 * long(n_);
-    apply_chunk(which, x + off, c, hy ? hy + off : nullptr, sy ? sy + off : nullptr, prec);
+    apply_chunk(which, x + off, c, hy ? hy + off : nullptr, sy ? sy + off : nullptr, prec,
+                owned_staging(which, c, prec));
   }
   if (which != 1 && ctx_.take_fault(Context::kFaultNonfiniteApply)) {
     // injected fault: all-ones bits are a NaN double
     check(hipMemsetAsync(hy, 0xFF, sizeof(double), ctx_.stream()), "fault injection");
+  }
+}
+
+namespace {
+std::size_t align256(std::size_t b) { return (b + 255) / 256 * 256; }
+}  // namespace
+
+std::size_t DeviceOperator::StagingBytes::total() const {
+  return 2 * align256(sticks) + align256(p) + align256(w) + align256(work);
+}
+
+// The buffers apply_chunk touches for `c` columns: stick buffers only when H
+// is applied, P when it is formed (atom-block and dense forms), W for any
+// nonlocal part, and the largest split-K workspace of the pass's GEMMs.
+DeviceOperator::StagingBytes DeviceOperator::staging_bytes(int which, long c, Precision prec) const {
+  StagingBytes z;
+  const long n = long(n_), np = long(np_);
+  const std::size_t csz = prec == Precision::FP64 ? 16 : 8;
+  if (which != 1) z.sticks = std::size_t(c) * std::size_t(grid_.dev.nsticks) * std::size_t(shape_[2]) * csz;
+  if (!np) return z;
+  const int cc = int(c);
+  z.w = 32 * std::size_t(np) * std::size_t(c);
+  if (form_ == 3 || form_ == 2) {
+    z.p = 16 * std::size_t(np) * std::size_t(c);
+    z.work = zgemm_workspace_bytes(int(np), cc, int(n));
+  }
+  if (form_ != 3) z.work = std::max(z.work, zgemm_workspace_bytes(int(np), cc, int(form_ == 2 ? np : n)));
+  z.work = std::max(z.work, zgemm_workspace_bytes(int(n), cc, int(np)));
+  return z;
+}
+
+DeviceOperator::Staging DeviceOperator::owned_staging(int which, long c, Precision prec) {
+  const StagingBytes z = staging_bytes(which, c, prec);
+  Staging st;
+  if (z.sticks) {
+    st.t1 = t1_.reserve(z.sticks);
+    st.t2 = t2_.reserve(z.sticks);
+  }
+  if (z.p) st.p = p_.reserve(z.p / sizeof(double));
+  if (z.w) st.w = w_.reserve(z.w / sizeof(double));
+  if (z.work) {
+    st.work = ctx_.gemm_work(z.work);
+    st.work_bytes = ctx_.gemm_work_bytes();
+  }
+  return st;
+}
+
+std::size_t DeviceOperator::noalloc_bytes(int which, long columns, Precision prec) const {
+  return staging_bytes(which, columns, prec).total();
+}
+
+void DeviceOperator::apply_noalloc(int which, const double* x, long columns, double* hy, double* sy, Precision prec) {
+  if (columns <= 0 || !x) throw std::invalid_argument("HIP: invalid plane-wave block operator buffers");
+  if ((which != 1 && (!hy || hy == x)) || (which != 0 && (!sy || sy == x)) || (which == 2 && hy == sy))
+    throw std::invalid_argument("HIP: invalid direct plane-wave input/output buffers");
+  if (rs_enabled_ && prec == Precision::FP32)
+    throw std::invalid_argument("HIP: the allocation-free apply does not take the real-space projector search");
+  char* base = ctx_.scratch();
+  if (!base) throw std::logic_error("HIP: context without a scratch array");
+  const std::size_t one = noalloc_bytes(which, 1, prec);
+  if (one > kScratchBytes)
+    throw std::overflow_error("HIP: one column of this operator needs " + std::to_string(one) +
+                            " bytes of staging, more than the " + std::to_string(kScratchBytes) +
+                            "-byte scratch array of the allocation-free apply");
+  for (long c0 = 0; c0 < columns;) {
+    // the most columns (up to apply()'s chunk) whose staging fits
+    long c = std::min<long>(long(chunk_), columns - c0);
+    while (c > 1 && noalloc_bytes(which, c, prec) > kScratchBytes) {
+      const double ratio = double(kScratchBytes) / double(noalloc_bytes(which, c, prec));
+      c = std::max<long>(1, std::min<long>(c - 1, long(double(c) * ratio)));
+    }
+    const StagingBytes z = staging_bytes(which, c, prec);
+    Staging st;
+    char* q = base;
+    auto carve = [&q](std::size_t bytes) {
+      char* r = bytes ? q : nullptr;
+      q += align256(bytes);
+      return r;
+    };
+    st.t1 = carve(z.sticks);
+    st.t2 = carve(z.sticks);
+    st.p = reinterpret_cast<double*>(carve(z.p));
+    st.w = reinterpret_cast<double*>(carve(z.w));
+    st.work = reinterpret_cast<double*>(carve(z.work));
+    st.work_bytes = z.work;
+    const long off = 2 * c0 * long(n_);
+    apply_chunk(which, x + off, c, hy ? hy + off : nullptr, sy ? sy + off : nullptr, prec, st);
+    c0 += c;
   }
 }
 
@@ -1043,7 +1143,8 @@ void DeviceOperator::projector_metric(const double* p, long columns, double* qp)
 // src/plane_wave_operator.cpp:218-374: local part through the stick pipeline,
 // P = B X once for the pair, onsite D/Q blocks, then B^H (.) added with the
 // kinetic/local H or the (1+reg) identity of S folded into the GEMM epilogue.
-void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy, double* sy, Precision prec) {
+void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy, double* sy, Precision prec,
+                                 const Staging& st) {
   hipStream_t s = ctx_.stream();
   const long n = long(n_), np = long(np_);
   const bool want_h = which != 1, want_s = which != 0;
@@ -1054,70 +1155,60 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
   if (want_h) {
     hipStream_t ls = overlap ? ctx_.aux_stream() : s;
     if (overlap) ctx_.fork();
-    const std::size_t bytes = std::size_t(c) * grid_.dev.nsticks * shape_[2] * (prec == Precision::FP64 ? 16 : 8);
-    t1_.reserve(bytes);
-    t2_.reserve(bytes);
     const bool f64 = prec == Precision::FP64;
     const double local_bytes = local_algorithmic_bytes(n, count_, c, f64);
     const LaunchEvents ev = ctx_.clock_launch(f64 ? Context::kLocalFP64 : Context::kLocalFP32, local_bytes, local_bytes);
     if (ev.begin) check(hipEventRecord(ev.begin, ls), "hipEventRecord");
     if (f64)
-      check(launch_local_h<double>(grid_.dev, x, n, c, kinetic_d_.p, hy, n, t1_.p, t2_.p, ls), "local H");
+      check(launch_local_h<double>(grid_.dev, x, n, c, kinetic_d_.p, hy, n, st.t1, st.t2, ls), "local H");
     else
-      check(launch_local_h<float>(grid_.dev, x, n, c, kinetic_d_.p, hy, n, t1_.p, t2_.p, ls), "local H");
+      check(launch_local_h<float>(grid_.dev, x, n, c, kinetic_d_.p, hy, n, st.t1, st.t2, ls), "local H");
     if (ev.end) check(hipEventRecord(ev.end, ls), "hipEventRecord");
   }
+  auto gemm = [&](int aop, long M, long K, double alpha, const double* A, long lda, const double* B, long ldb,
+                  double* C, long ldc, const double* D, long ldd, double beta) {
+    ctx_.zgemm_into(prec, aop, 0, M, c, K, alpha, A, lda, B, ldb, C, ldc, D, ldd, beta, st.work, st.work_bytes);
+  };
   if (np && form_ != 3) {
     // PWO2 dense D/Q on P = B X, or PWO1 folded DB/QB on X
     // (src/plane_wave_operator.cpp:290-301)
-    w_.reserve(std::size_t(2 * np * 2 * c));
     const bool dense = form_ == 2;
-    if (dense) {
-      p_.reserve(std::size_t(2 * np * c));
-      ctx_.zgemm(prec, 0, 0, np, c, n, 1.0, b64_.p, np, x, n, p_.p, np, nullptr, 0, 0.0);
-    }
-    const double* rhs = dense ? p_.p : x;
+    if (dense) gemm(0, np, n, 1.0, b64_.p, np, x, n, st.p, np, nullptr, 0, 0.0);
+    const double* rhs = dense ? st.p : x;
     const long inner = dense ? np : n, ldr = dense ? np : n;
-    if (want_h) ctx_.zgemm(prec, 0, 0, np, c, inner, 1.0, onsite_d_.p, np, rhs, ldr, w_.p, np, nullptr, 0, 0.0);
-    if (want_s)
-      ctx_.zgemm(prec, 0, 0, np, c, inner, 1.0, onsite_q_.p, np, rhs, ldr, w_.p + 2 * np * c, np, nullptr, 0, 0.0);
+    if (want_h) gemm(0, np, inner, 1.0, onsite_d_.p, np, rhs, ldr, st.w, np, nullptr, 0, 0.0);
+    if (want_s) gemm(0, np, inner, 1.0, onsite_q_.p, np, rhs, ldr, st.w + 2 * np * c, np, nullptr, 0, 0.0);
   } else if (np) {
-    p_.reserve(std::size_t(2 * np * c));
-    w_.reserve(std::size_t(2 * np * 2 * c));
     if (prec == Precision::FP64)
-      ctx_.zgemm(prec, 0, 0, np, c, n, 1.0, b64_.p, np, x, n, p_.p, np, nullptr, 0, 0.0);
+      gemm(0, np, n, 1.0, b64_.p, np, x, n, st.p, np, nullptr, 0, 0.0);
     else {
-      const std::size_t bytes = zgemm_workspace_bytes(int(np), int(c), int(n));
-      double* wk = bytes ? ctx_.gemm_work(bytes) : nullptr;
       const LaunchEvents ev =
           ctx_.clock_launch(Context::kGemmFP32, 8.0 * np * c * n, gemm_algorithmic_bytes(np, c, n, 8, false));
-      check(zgemm_f32_storage(0, 0, int(np), int(c), int(n), 1.0, b32_.p, np, x, n, p_.p, np, nullptr, 0, 0.0, wk,
-                              bytes ? ctx_.gemm_work_bytes() : 0, s, &ev),
+      check(zgemm_f32_storage(0, 0, int(np), int(c), int(n), 1.0, b32_.p, np, x, n, st.p, np, nullptr, 0, 0.0, st.work,
+                              st.work_bytes, s, &ev),
             "projection");
     }
     const int mask = (want_h ? 1 : 0) | (want_s ? 2 : 0);
     check(launch_atom_blocks(prec, int(np), int(c), atoms_, atom_offset_.p, atom_rows_.p, block_offset_.p, dblk_.p,
-                             qblk_.p, p_.p, np, w_.p, np, mask, s),
+                             qblk_.p, st.p, np, st.w, np, mask, s),
           "onsite blocks");
   }
   auto backproject = [&](const double* w, double* out, const double* d, double beta) {
     if (prec == Precision::FP64)
-      ctx_.zgemm(prec, 1, 0, n, c, np, 1.0, b64_.p, np, w, np, out, n, d, n, beta);
+      gemm(1, n, np, 1.0, b64_.p, np, w, np, out, n, d, n, beta);
     else {
-      const std::size_t bytes = zgemm_workspace_bytes(int(n), int(c), int(np));
-      double* wk = bytes ? ctx_.gemm_work(bytes) : nullptr;
       const LaunchEvents ev =
           ctx_.clock_launch(Context::kGemmFP32, 8.0 * n * c * np, gemm_algorithmic_bytes(n, c, np, 8, beta != 0.0));
-      check(zgemm_f32_storage(1, 0, int(n), int(c), int(np), 1.0, b32_.p, np, w, np, out, n, d, n, beta, wk,
-                              bytes ? ctx_.gemm_work_bytes() : 0, s, &ev),
+      check(zgemm_f32_storage(1, 0, int(n), int(c), int(np), 1.0, b32_.p, np, w, np, out, n, d, n, beta, st.work,
+                              st.work_bytes, s, &ev),
             "back projection");
     }
   };
   if (np) {
-    if (want_s) backproject(w_.p + 2 * np * c, sy, x, 1.0 + reg_);
+    if (want_s) backproject(st.w + 2 * np * c, sy, x, 1.0 + reg_);
     if (want_h) {
       if (overlap) ctx_.join();
-      backproject(w_.p, hy, hy, 1.0);
+      backproject(st.w, hy, hy, 1.0);
     }
   } else if (want_s) {
     check(launch_scale_copy(x, n, sy, n, n, c, 1.0 + reg_, s), "overlap identity");

@@ -130,6 +130,13 @@ class Context {
   // C = alpha op(A) op(B) + beta D on device pointers, workspace managed here.
   void zgemm(Precision prec, int aop, int bop, long M, long N, long K, double alpha, const double* A, long lda,
              const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta);
+  // The same with the split-K workspace supplied by the caller (work_bytes
+  // >= zgemm_workspace_bytes(M, N, K) keeps the plan zgemm above would take).
+  void zgemm_into(Precision prec, int aop, int bop, long M, long N, long K, double alpha, const double* A, long lda,
+                  const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta, double* work,
+                  std::size_t work_bytes);
+  // this device's statically allocated scratch array (kScratchBytes)
+  char* scratch() const { return scratch_; }
   void record(const std::string& name, double seconds, std::size_t calls = 1);
   std::map<std::string, PerfCounter> counters;
 
@@ -191,6 +198,7 @@ class Context {
   void bulk_buffers();
   rocblas_handle blas_{};
   std::string name_;
+  char* scratch_ = nullptr;
   DevBuf<double> work_;
   bool clock_on_ = false;
   std::vector<hipEvent_t> events_;
@@ -241,6 +249,15 @@ class DeviceOperator {
                           const std::complex<double>* initial) ;
   // which: 0 = H, 1 = S, 2 = joint pair. x, hy, sy: device pointers, ld = n.
   void apply(int which, const double* x, long columns, double* hy, double* sy, Precision prec);
+  // The same application staged entirely in the context's static scratch
+  // array: no device allocation. Columns go in passes of the most that fit
+  // (at most the chunk apply() uses, so a block apply() takes in one pass
+  // gives identical results here when it fits). Throws std::overflow_error
+  // when one column does not fit, std::invalid_argument for FP32 while the
+  // real-space projector search is enabled (its staging is not bounded here).
+  void apply_noalloc(int which, const double* x, long columns, double* hy, double* sy, Precision prec);
+  // scratch bytes apply_noalloc needs for one pass of `columns` columns
+  std::size_t noalloc_bytes(int which, long columns, Precision prec) const;
   // P = B X (np x columns, ld np) and QP = Q P with the onsite metric
   // (src/plane_wave_operator.cpp:375-389); device pointers.
   void project(const double* x, long columns, double* p, Precision prec);
@@ -306,7 +323,25 @@ class DeviceOperator {
   void* stick_buffer(int which, std::size_t columns, Precision prec);
 
  private:
-  void apply_chunk(int which, const double* x, long columns, double* hy, double* sy, Precision prec);
+  // Device staging of one apply_chunk pass: the two stick buffers of the
+  // local-Hamiltonian pipeline, P = B X, W = (D P | Q P) and the split-K
+  // partials of the pass's GEMMs.
+  struct Staging {
+    void* t1 = nullptr;
+    void* t2 = nullptr;
+    double* p = nullptr;
+    double* w = nullptr;
+    double* work = nullptr;
+    std::size_t work_bytes = 0;
+  };
+  struct StagingBytes {
+    std::size_t sticks = 0, p = 0, w = 0, work = 0;
+    std::size_t total() const;  // with 256-byte alignment of each region
+  };
+  StagingBytes staging_bytes(int which, long columns, Precision prec) const;
+  Staging owned_staging(int which, long columns, Precision prec);
+  void apply_chunk(int which, const double* x, long columns, double* hy, double* sy, Precision prec,
+                   const Staging& st);
   std::shared_ptr<void> davidson_cache_;
   Context& ctx_;
   std::size_t n_ = 0, np_ = 0, chunk_ = 256;

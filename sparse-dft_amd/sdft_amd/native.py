@@ -532,6 +532,22 @@ class Operator:
                                              ct.c_int(precision), ct.c_int(0)))
         return {"h": hy, "s": sy, "pair": (hy, sy)}[which]
 
+    def apply_device(self, x_ptr, columns, hy_ptr, sy_ptr, which="pair", precision=0, allocate=True):
+        """H x and/or S x on device buffers (integer device addresses of n x
+        columns complex column-major blocks), returning when they are written.
+        allocate=False takes the allocation-free entry point, which stages
+        everything in the library's fixed per-GPU scratch array."""
+        code = {"h": 0, "s": 1, "pair": 2}[which]
+        hy = ct.c_void_p(hy_ptr) if code != 1 else None
+        sy = ct.c_void_p(sy_ptr) if code != 0 else None
+        if allocate:
+            rc = lib().sdft_hip_operator_apply(self.handle, ct.c_int(code), ct.c_void_p(x_ptr), ct.c_size_t(columns),
+                                               hy, sy, ct.c_int(precision), ct.c_int(1))
+        else:
+            rc = lib().sdft_hip_operator_apply_noalloc(self.handle, ct.c_int(code), ct.c_void_p(x_ptr),
+                                                       ct.c_size_t(columns), hy, sy, ct.c_int(precision))
+        _check(rc)
+
     def benchmark(self, columns, which="pair", precision=0, repetitions=10):
         ms = np.zeros(3)
         code = {"h": 0, "s": 1, "pair": 2}[which]
