@@ -114,13 +114,25 @@ def test_mp_batch_concurrent_matches_sequential():
         svc.close()
     assert all(r["converged"] for r in results)
     assert max(r["residual_norms"].max() for r in results) <= 1e-12
+    print(f"concurrent k-point solves: {sum(r['rr_barrier_fallbacks'] for r in results)} Rayleigh-Ritz "
+          f"fallbacks in {sum(r['iterations'] for r in results)} iterations")
     ctx = native.Context(0)
     try:
         for i in (0, 17, 42, 63):
             dev = native.Operator(ctx, probs[i])
             r = dev.davidson(32, 256, 1e-12, inits[i])
-            assert np.array_equal(r["eigenvalues"], results[i]["eigenvalues"])
-            assert np.array_equal(r["eigenvectors"], results[i]["eigenvectors"])
+            fallbacks = (r["rr_barrier_fallbacks"], results[i]["rr_barrier_fallbacks"])
+            if fallbacks == (0, 0):
+                assert np.array_equal(r["eigenvalues"], results[i]["eigenvalues"]), (i, fallbacks)
+                assert np.array_equal(r["eigenvectors"], results[i]["eigenvectors"]), (i, fallbacks)
+            else:
+                # a Rayleigh-Ritz solve redone by zheevd (its reduction's bounded
+                # wait expired while other contexts held the CUs) rounds
+                # differently: both solves are certified, so they agree to the
+                # certificate's accuracy
+                assert np.abs(r["eigenvalues"] - results[i]["eigenvalues"]).max() < 1e-10, (i, fallbacks)
+                x, y = r["eigenvectors"], results[i]["eigenvectors"]
+                assert np.linalg.norm(x @ x.conj().T - y @ y.conj().T, 2) < 1e-6, (i, fallbacks)
             dev.close()
     finally:
         ctx.close()
