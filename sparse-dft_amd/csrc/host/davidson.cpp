@@ -691,8 +691,13 @@ void Solver::preserved_subspace(long keep, double* retained) {
 // block W = [X, G] holds the R wanted Ritz vectors X and R guard Ritz vectors
 // G (the next ones up), columns 0..2R of basis_ with their images in hb_ and
 // sb_. Each step:
-//   1. fresh FP64 H and S images of W; Rayleigh quotients, S normalisation
-//      and residuals of X; when every root meets the tolerance, certify();
+//   1. FP64 H and S images of W: fresh applications on the first step (W
+//      comes from the search subspace), afterwards the FP64 images of [W, T]
+//      rotated by the Rayleigh-Ritz vectors as the Davidson rotates its
+//      images (two GEMMs instead of a 2R-column pair application; the
+//      certificate applies H and S afresh either way); Rayleigh quotients, S
+//      normalisation and residuals of X; when every root meets the
+//      tolerance, certify();
 //   2. corrections T for the unconverged roots from the correction equations
 //      P^H (H - theta_j S) P t_j = r_j, P = I - X (S X)^H, by preconditioned
 //      CG in search (FP32) arithmetic (correction_cg);
@@ -780,12 +785,18 @@ bool Solver::refine(long& it, long& iterations) {
   double best = std::numeric_limits<double>::infinity();
   int stalls = 0;
   double* wbuf = retained_.p;
+  long rotated_at = -1;  // >= 0: W's rotated images wait in hb_/sb_ columns from there
   for (; it + 1 < p_.max_iterations;) {
     ++it;
     iterations = it + 1;
-    // 1. fresh FP64 images of W; X = its first R columns
+    // 1. FP64 images of W; X = its first R columns
     copy(basis_.p, wbuf, n * k);
-    apply(2, basis_.p, k, hb_.p, sb_.p, true);
+    if (rotated_at >= 0) {
+      copy(hb_.p, col(hb_, rotated_at, n), n * k);
+      copy(sb_.p, col(sb_, rotated_at, n), n * k);
+    } else {
+      apply(2, basis_.p, k, hb_.p, sb_.p, true);
+    }
     const std::vector<double> metric = real_dots(basis_.p, sb_.p, R);
     const std::vector<double> energy = real_dots(basis_.p, hb_.p, R);
     std::vector<double> root(R);
@@ -866,6 +877,12 @@ bool Solver::refine(long& it, long& iterations) {
     const std::vector<double> w = rr_solve(A, B, m, k);
     for (long r = 0; r < R; ++r) values_[r] = w[r];
     gemm(true, 0, 0, n, k, m, 1.0, basis_.p, n, A, m, wbuf, n);
+    // the next W's images, past the m columns in use when the buffers have room
+    rotated_at = m + k <= cap_ ? m : -1;
+    if (rotated_at >= 0) {
+      gemm(true, 0, 0, n, k, m, 1.0, hb_.p, n, A, m, col(hb_, m, n), n);
+      gemm(true, 0, 0, n, k, m, 1.0, sb_.p, n, A, m, col(sb_, m, n), n);
+    }
   }
   copy(x_.p, wbuf, n * R);
   return false;
