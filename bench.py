@@ -206,11 +206,13 @@ def main():
     ap.add_argument("--refinement", type=int, default=1,
                     help="FP32 search after the hand-off: 1 = block Jacobi-Davidson refinement (FP32 CG correction "
                          "solves, FP64 images and Rayleigh-Ritz); 0 = the reference's FP64 Davidson iterations")
-    ap.add_argument("--inner-iterations", type=int, default=6, help="CG steps per refinement correction solve")
-    ap.add_argument("--search-handoff", type=float, default=3e-3,
-                    help="max residual at which the FP32 search hands off (the reference hands off at 1e-8; with the "
-                         "Jacobi-Davidson refinement 3e-3 measured fastest: 1e-2 / 3e-3 / 1e-3 / 1e-4 / 1e-8 gave "
-                         "621 / 621 / 591-620 / 555 / 535 certified eigenpairs/s)")
+    ap.add_argument("--inner-iterations", type=int, default=6,
+                    help="CG steps per refinement correction solve (hand-off 1: 4 / 6 / 8 gave 691 / 716 / 711)")
+    ap.add_argument("--search-handoff", type=float, default=1.0,
+                    help="max residual at which the FP32 search hands off (the reference hands off at 1e-8). With the "
+                         "Jacobi-Davidson refinement taking its images by rotation, the earliest hand-off measured "
+                         "fastest: 1 (after the second Rayleigh-Ritz) / 3e-1 / 1e-1 / 3e-2 / 3e-3 gave 714-716 / 703 / "
+                         "667 / 692 / 647 certified eigenpairs/s, and 10 (after the first) 685")
     ap.add_argument("--serving", type=int, default=60,
                     help="serving leg: N requests (Si8..Si64 mix, Poisson arrivals at 0.7 x saturation) through the "
                          "serving engine, p50/p99 of arrival -> certified result (0: skip)")

@@ -31,10 +31,11 @@ def certify(h, s, r, roots):
 
 
 @pytest.mark.parametrize("search_precision,refinement,handoff", [(0, 0, 1e-8), (1, 0, 1e-8), (1, 1, 1e-8),
-                                                                   (1, 1, 3e-3)])
+                                                                   (1, 1, 3e-3), (1, 1, 1.0)])
 def test_davidson_si8_certified(si8, si8_dev, si8_dense, search_precision, refinement, handoff):
     """FP64 path, FP32 search with the reference's FP64 Davidson tail, and FP32
-    search with the Jacobi-Davidson refinement (hand-off at 1e-8 and 3e-3)."""
+    search with the Jacobi-Davidson refinement (hand-off at 1e-8, 3e-3 and 1, the
+    last handing off after the first Rayleigh-Ritz steps)."""
     h, s, w = si8_dense
     init = syn.initial_psi(si8, 32, 0)
     r = si8_dev.davidson(16, 256, 1e-12, init, search_precision=search_precision, refinement=refinement,
