@@ -206,8 +206,12 @@ def main():
     ap.add_argument("--refinement", type=int, default=1,
                     help="FP32 search after the hand-off: 1 = block Jacobi-Davidson refinement (FP32 CG correction "
                          "solves, FP64 images and Rayleigh-Ritz); 0 = the reference's FP64 Davidson iterations")
-    ap.add_argument("--inner-iterations", type=int, default=6,
-                    help="CG steps per refinement correction solve (hand-off 1: 4 / 6 / 8 gave 691 / 716 / 711)")
+    ap.add_argument("--inner-iterations", type=int, default=8,
+                    help="CG steps per refinement correction solve (with 60 guards: 6 / 8 gave 766 / 780; with 160: "
+                         "4 / 6 / 8 gave 691 / 716 / 711)")
+    ap.add_argument("--refinement-guards", type=int, default=60,
+                    help="guard Ritz vectors beside the 160 roots in the refinement block (with 8 CG steps: "
+                         "32 / 60 / 80 / 160 gave 746 / 780 / 766 / 711; 0 = roots / 2)")
     ap.add_argument("--search-handoff", type=float, default=1.0,
                     help="max residual at which the FP32 search hands off (the reference hands off at 1e-8). With the "
                          "Jacobi-Davidson refinement taking its images by rotation, the earliest hand-off measured "
@@ -252,6 +256,7 @@ def main():
                         timing=args.timing, subspace_factor=args.subspace_factor,
                         stagnation_window=args.stagnation_window, handoff_restart=args.handoff_restart,
                         refinement=args.refinement if precision else 0, inner_iterations=args.inner_iterations,
+                        refinement_guards=args.refinement_guards,
                         search_handoff=args.search_handoff)
         ctx.synchronize()
         r["wall"] = time.perf_counter() - t0
@@ -413,6 +418,7 @@ def main():
                                                              stagnation_window=args.stagnation_window,
                                                              refinement=args.refinement,
                                                              inner_iterations=args.inner_iterations,
+                                                             refinement_guards=0,
                                                              search_handoff=args.search_handoff))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prob)

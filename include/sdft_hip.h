@@ -207,6 +207,9 @@ typedef struct {
    * Ritz block and the corrections. The certificate is the same. */
   int refinement;
   int inner_iterations; /* refinement: CG steps per correction solve, 0 = default */
+  /* refinement: guard Ritz vectors carried beside the roots in its block
+   * (1 .. roots; 0 = roots / 2) */
+  long refinement_guards;
 } sdft_hip_davidson_params;
 
 typedef struct {

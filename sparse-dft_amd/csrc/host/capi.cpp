@@ -575,8 +575,11 @@ int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* par
     p.band_group_size = params->band_group_size;
     p.refinement = params->refinement;
     p.inner_iterations = params->inner_iterations;
+    p.refinement_guards = params->refinement_guards;
     if (p.refinement != 0 && p.refinement != 1) throw std::invalid_argument("HIP: refinement must be 0 or 1");
     if (p.inner_iterations < 0) throw std::invalid_argument("HIP: inner iterations must be >= 0");
+    if (p.refinement_guards < 0 || p.refinement_guards > params->roots)
+      throw std::invalid_argument("HIP: refinement guards must be in 0 .. roots");
     p.vectors_out = reinterpret_cast<std::complex<double>*>(result->eigenvectors);
     if (p.search_fp32 != 0 && p.search_fp32 != 1) throw std::invalid_argument("HIP: search precision must be 0 or 1");
     const auto out = sdft_hip::davidson(op->impl, p, reinterpret_cast<const std::complex<double>*>(initial));

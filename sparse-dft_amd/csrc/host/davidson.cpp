@@ -123,6 +123,10 @@ class Solver {
 
  private:
   Precision search_prec() const { return exact_ ? Precision::FP64 : Precision::FP32; }
+  // guard Ritz vectors of the refinement block (R/2 unless set)
+  long refinement_guards() const {
+    return p_.refinement_guards > 0 ? std::min(p_.refinement_guards, R_) : std::max(1L, R_ / 2);
+  }
   Precision prec(bool exact) const { return exact ? Precision::FP64 : search_prec(); }
   double* col(DevBuf<double>& b, long j, long ld) { return b.p + 2 * j * ld; }
   void sync() {
@@ -688,13 +692,13 @@ void Solver::preserved_subspace(long keep, double* retained) {
 }
 
 // Jacobi-Davidson refinement after the hand-off from search arithmetic. The
-// block W = [X, G] holds the R wanted Ritz vectors X and R guard Ritz vectors
-// G (the next ones up), columns 0..2R of basis_ with their images in hb_ and
-// sb_. Each step:
+// block W = [X, G] holds the R wanted Ritz vectors X and the guard Ritz
+// vectors G (the next refinement_guards() up), columns 0..k of basis_ with
+// their images in hb_ and sb_. Each step:
 //   1. FP64 H and S images of W: fresh applications on the first step (W
 //      comes from the search subspace), afterwards the FP64 images of [W, T]
 //      rotated by the Rayleigh-Ritz vectors as the Davidson rotates its
-//      images (two GEMMs instead of a 2R-column pair application; the
+//      images (two GEMMs instead of a k-column pair application; the
 //      certificate applies H and S afresh either way); Rayleigh quotients, S
 //      normalisation and residuals of X; when every root meets the
 //      tolerance, certify();
@@ -702,7 +706,7 @@ void Solver::preserved_subspace(long keep, double* retained) {
 //      P^H (H - theta_j S) P t_j = r_j, P = I - X (S X)^H, by preconditioned
 //      CG in search (FP32) arithmetic (correction_cg);
 //   3. T projected against W and S-orthonormalised in FP64, with FP64 images;
-//   4. FP64 Rayleigh-Ritz of [W, T]; the lowest 2R Ritz vectors are the next W.
+//   4. FP64 Rayleigh-Ritz of [W, T]; the lowest k Ritz vectors are the next W.
 // The guards enter the Rayleigh-Ritz (roots can move across the block edge)
 // but not the projector: guards from the search subspace are far less
 // accurate than the wanted roots, and projecting them out distorts the
@@ -726,7 +730,7 @@ void Solver::correction_cg(const std::vector<int>& active, double* t) {
   double* sq = hq + 2 * n * R;
   double* z = w4_.p;
   double* res = z + 2 * n * R;
-  double* coef = m1_.p;  // 2R x cols
+  double* coef = m1_.p;  // R x cols
   double* base = small_.p + 8 + 4 * R;
   double* theta = base;
   double* wk = base + R;
@@ -778,9 +782,9 @@ void Solver::correction_cg(const std::vector<int>& active, double* t) {
   }
 }
 
-// W (2R columns, n x 2R) in retained_ on entry and exit.
+// W (k = R + guards columns, n x k) in retained_ on entry and exit.
 bool Solver::refine(long& it, long& iterations) {
-  const long n = n_, R = R_, k = 2 * R;
+  const long n = n_, R = R_, k = R + refinement_guards();
   const double t0 = now();
   double best = std::numeric_limits<double>::infinity();
   int stalls = 0;
@@ -984,13 +988,14 @@ DavidsonOutput Solver::run(const C* initial) {
       if (max_res() < p_.search_handoff || it + 1 == p_.max_iterations || stalled) {
         stats_.handoff_iteration = it + 1;
         search_residual_ = max_res();
-        if (p_.refinement && cap_ >= 4 * R && size_ >= 2 * R && p_.handoff_restart >= 1 &&
+        const long jd_block = R + refinement_guards();
+        if (p_.refinement && cap_ >= 4 * R && size_ >= jd_block && p_.handoff_restart >= 1 &&
             it + 1 < p_.max_iterations) {
-          // the lowest 2R Ritz vectors of the search subspace in FP64
+          // the lowest R + guards Ritz vectors of the search subspace in FP64
           // arithmetic, then the Jacobi-Davidson refinement; it returns false
           // to hand them to the FP64 Davidson iterations below
           exact_ = true;
-          gemm(true, 0, 0, n, 2 * R, size_, 1.0, basis_.p, n, A, size_, retained.p, n);
+          gemm(true, 0, 0, n, jd_block, size_, 1.0, basis_.p, n, A, size_, retained.p, n);
           if (refine(it, iterations)) {
             certified = true;
             break;
@@ -998,7 +1003,7 @@ DavidsonOutput Solver::run(const C* initial) {
           size_ = 0;
           computed_h_ = 0;
           projected_size_ = 0;
-          append_block(retained.p, 2 * R);
+          append_block(retained.p, jd_block);
           continue;
         }
         const long keep = std::min(size_, long(std::max(2, p_.handoff_restart)) * R);

@@ -27,6 +27,7 @@ struct DavidsonParams {
                                   // after the hand-off; 1 runs block Jacobi-Davidson refinement steps (FP32
                                   // correction solves, FP64 images and Rayleigh-Ritz)
   int inner_iterations = 0;       // refinement: preconditioned CG steps per correction solve (0: default)
+  long refinement_guards = 0;     // refinement: guard Ritz vectors beside the roots (0: roots / 2)
   int verbose = 0;
   int timing = 0;                 // synchronise phases and accumulate per-phase times
   long band_group_size = 0;       // > 0: bounded band-group solver (src/band_group_solver.cpp:57-211)

@@ -37,7 +37,7 @@ class DavidsonParams(ct.Structure):
                 ("subspace_factor", ct.c_long), ("search_precision", ct.c_int), ("search_handoff", ct.c_double),
                 ("stagnation_window", ct.c_int), ("handoff_restart", ct.c_int), ("verbose", ct.c_int),
                 ("timing", ct.c_int), ("band_group_size", ct.c_long), ("refinement", ct.c_int),
-                ("inner_iterations", ct.c_int)]
+                ("inner_iterations", ct.c_int), ("refinement_guards", ct.c_long)]
 
 
 class DavidsonResult(ct.Structure):
@@ -557,10 +557,11 @@ class Operator:
 
     def davidson(self, roots, max_iterations=256, tolerance=1e-12, initial=None, subspace_factor=8,
                  search_precision=0, search_handoff=1e-8, stagnation_window=3, handoff_restart=1, verbose=0,
-                 timing=0, want_vectors=True, band_group_size=0, refinement=0, inner_iterations=0):
+                 timing=0, want_vectors=True, band_group_size=0, refinement=0, inner_iterations=0,
+                 refinement_guards=0):
         p = DavidsonParams(roots, max_iterations, tolerance, subspace_factor, search_precision, search_handoff,
                            stagnation_window, handoff_restart, verbose, timing, band_group_size, refinement,
-                           inner_iterations)
+                           inner_iterations, refinement_guards)
         ev = np.zeros(roots)
         res = np.zeros(roots)
         vec = np.zeros((self.n, roots), dtype=np.complex128, order="F") if want_vectors else None

@@ -80,7 +80,9 @@ def test_si64_davidson_against_reference(hip_ctx):
             dev.davidson(160, 256, 1e-12, init, subspace_factor=5, search_precision=1, refinement=1,
                          search_handoff=3e-3),
             dev.davidson(160, 256, 1e-12, init, subspace_factor=5, search_precision=1, refinement=1,
-                         search_handoff=1.0, inner_iterations=6)]
+                         search_handoff=1.0, inner_iterations=6),
+            dev.davidson(160, 256, 1e-12, init, subspace_factor=5, search_precision=1, refinement=1,
+                         search_handoff=1.0, inner_iterations=8, refinement_guards=60)]
     for r in runs:
         assert r["converged"]
         assert fresh_residuals(dev, r).max() <= 1.01e-12
