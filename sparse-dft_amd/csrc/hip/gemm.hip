@@ -16,7 +16,7 @@
 namespace sdft_hip {
 
 namespace {
-constexpr int BK = 16, PAD = 4;
+constexpr int BK = 16;  // K slab per LDS round
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -88,8 +88,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
   constexpr int WGN = 4 / WGM, WM = TBM / WGM, WN = TBN / WGN, IM = WM / 16, JN = WN / 16;
   constexpr int EA = TBM * BK / 256, EB = TBN * BK / 256;
   static_assert(EA * 256 == TBM * BK && EB * 256 == TBN * BK, "staging must tile the workgroup");
-  __shared__ T as_re[BK][TBM + PAD], as_im[BK][TBM + PAD];
-  __shared__ T bs_re[BK][TBN + PAD], bs_im[BK][TBN + PAD];
+  // LDS staging, conflict-free in both directions: an operand whose global
+  // layout runs along K (adjoint A, normal B) is loaded by lanes along K and
+  // kept [m][k] with row stride BK + 1; the other one is loaded along m and
+  // kept [k][m] with row stride TBM + 16 (two K rows of one half-wave read
+  // land on disjoint banks).
+  constexpr int SA = ADJ_A ? BK + 1 : TBM + 16, SB = ADJ_B ? TBN + 16 : BK + 1;
+  constexpr int LA = ADJ_A ? TBM * SA : BK * SA, LB = ADJ_B ? BK * SB : TBN * SB;
+  __shared__ T as_re[LA], as_im[LA];
+  __shared__ T bs_re[LB], bs_im[LB];
+  auto aix = [](int kk, int mm) { return ADJ_A ? mm * SA + kk : kk * SA + mm; };
+  auto bix = [](int kk, int nn) { return ADJ_B ? kk * SB + nn : nn * SB + kk; };
   using V = typename Acc<T>::type;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const TileId tile = xcd_tile(gridDim.x, gridDim.y, gridDim.z);
@@ -150,16 +159,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       const int e = tid + it * 256;
       int mm, kk;
       if (!ADJ_A) { mm = e % TBM; kk = e / TBM; } else { kk = e % BK; mm = e / BK; }
-      as_re[kk][mm] = T(pa[it][0]);
-      as_im[kk][mm] = T(pa[it][1]);
+      as_re[aix(kk, mm)] = T(pa[it][0]);
+      as_im[aix(kk, mm)] = T(pa[it][1]);
     }
 #pragma unroll
     for (int it = 0; it < EB; ++it) {
       const int e = tid + it * 256;
       int nn, kk;
       if (!ADJ_B) { kk = e % BK; nn = e / BK; } else { nn = e % TBN; kk = e / TBN; }
-      bs_re[kk][nn] = T(pb[it][0]);
-      bs_im[kk][nn] = T(pb[it][1]);
+      bs_re[bix(kk, nn)] = T(pb[it][0]);
+      bs_im[bix(kk, nn)] = T(pb[it][1]);
     }
   };
   if (PF && k0 < k1) load(k0);
@@ -174,13 +183,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       T ar[IM], ai[IM], br[JN], bi[JN];
 #pragma unroll
       for (int i = 0; i < IM; ++i) {
-        ar[i] = as_re[kr][wm + 16 * i + c];
-        ai[i] = as_im[kr][wm + 16 * i + c];
+        ar[i] = as_re[aix(kr, wm + 16 * i + c)];
+        ai[i] = as_im[aix(kr, wm + 16 * i + c)];
       }
 #pragma unroll
       for (int j = 0; j < JN; ++j) {
-        br[j] = bs_re[kr][wn + 16 * j + c];
-        bi[j] = bs_im[kr][wn + 16 * j + c];
+        br[j] = bs_re[bix(kr, wn + 16 * j + c)];
+        bi[j] = bs_im[bix(kr, wn + 16 * j + c)];
       }
       if constexpr (kGauss) {
         T as[IM], bsum[JN];
