@@ -217,6 +217,15 @@ __global__ void k_scale_copy(const double* __restrict__ x, long ldx, double* __r
     y[2 * col * ldy + i] = scale * x[2 * col * ldx + i];
 }
 
+// y(:, j) += scale * theta[j] * x(:, j), complex columns of `rows` entries.
+__global__ void k_column_scaled_add(const double* __restrict__ x, long ldx, double* __restrict__ y, long ldy,
+                                    long rows, const double* __restrict__ theta, double scale) {
+  const long col = blockIdx.y;
+  const double f = scale * theta[col];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < 2 * rows; i += (long)gridDim.x * blockDim.x)
+    y[2 * col * ldy + i] += f * x[2 * col * ldx + i];
+}
+
 // Potential setup: natural-order complex grid -> real slot-order V (FP64 and
 // FP32 copies) plus the largest |Im| / max(1, |Re|) ratio (block maxima).
 __global__ void k_potential_finish(const cx<double>* __restrict__ planes, long count, double* __restrict__ v64,
@@ -319,6 +328,17 @@ template hipError_t launch_local_h<double>(const GridPlanDev&, const double*, lo
                                            void*, void*, hipStream_t);
 template hipError_t launch_local_h<float>(const GridPlanDev&, const double*, long, long, const double*, double*, long,
                                           void*, void*, hipStream_t);
+
+hipError_t launch_column_scaled_add(const double* x, long ldx, double* y, long ldy, long rows, long columns,
+                                    const double* theta, double scale, hipStream_t stream) {
+  if (rows <= 0 || columns <= 0) return hipSuccess;
+  if (columns > 65535) return hipErrorInvalidValue;
+  const long blocks = (2 * rows + 255) / 256;
+  const unsigned bx = unsigned(blocks < 256 ? blocks : 256);
+  hipLaunchKernelGGL(k_column_scaled_add, dim3(bx, unsigned(columns)), dim3(256), 0, stream, x, ldx, y, ldy, rows,
+                     theta, scale);
+  return hipGetLastError();
+}
 
 hipError_t launch_scale_copy(const double* x, long ldx, double* y, long ldy, long rows, long columns, double scale,
                              hipStream_t stream) {

@@ -26,6 +26,9 @@ template <typename T>
 hipError_t launch_local_h(const GridPlanDev& p, const double* x, long ldx, long columns, const double* kinetic,
                           double* y, long ldy, void* t1, void* t2, hipStream_t stream);
 
+// y(:, j) += scale * theta[j] * x(:, j) for j < columns (theta on the device)
+hipError_t launch_column_scaled_add(const double* x, long ldx, double* y, long ldy, long rows, long columns,
+                                    const double* theta, double scale, hipStream_t stream);
 hipError_t launch_scale_copy(const double* x, long ldx, double* y, long ldy, long rows, long columns, double scale,
                              hipStream_t stream);
 hipError_t launch_potential_setup(const GridPlanDev& full, const double* grid, void* t1, void* t2, double* v64,

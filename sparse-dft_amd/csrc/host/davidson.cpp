@@ -726,8 +726,6 @@ void Solver::correction_cg(const std::vector<int>& active, double* t) {
   const double* sw = sb_.p;
   double* p = w3_.p;
   double* q = p + 2 * n * R;
-  double* hq = q + 2 * n * R;
-  double* sq = hq + 2 * n * R;
   double* z = w4_.p;
   double* res = z + 2 * n * R;
   double* coef = m1_.p;  // R x cols
@@ -756,9 +754,9 @@ void Solver::correction_cg(const std::vector<int>& active, double* t) {
   copy(p, z, n * cols);
   check(launch_real_dots(res, n, z, n, n, cols, rho0, s_), "dots");
   for (int it = 0; it < steps; ++it) {
-    op_.apply(2, p, cols, hq, sq, f32);
+    // q = (H - theta S) p in one application (a single back-projection)
+    op_.apply_shifted(p, cols, theta, q, f32);
     stats_.h_columns_search += cols;
-    check(launch_fma_residual(hq, sq, n, n, cols, theta, q, s_), "shifted");
     project(q, true);
     check(launch_real_dots(p, n, q, n, n, cols, pq, s_), "dots");
     check(launch_cg_update(t, res, p, q, n, n, cols, rho0, pq, s_), "cg update");

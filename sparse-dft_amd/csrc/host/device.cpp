@@ -738,6 +738,29 @@ void DeviceOperator::apply(int which, const double* x, long columns, double* hy,
   }
 }
 
+void DeviceOperator::apply_shifted(const double* x, long columns, const double* theta, double* y, Precision prec) {
+  if (columns <= 0 || !x || !y || y == x || !theta)
+    throw std::invalid_argument("HIP: invalid shifted plane-wave application buffers");
+  if (rs_enabled_ && prec == Precision::FP32) {
+    // the real-space projectors apply H and S separately
+    double* sx = shift_work_.reserve(2 * n_ * std::size_t(columns));
+    apply_real_space(0, x, columns, y, prec);
+    apply_real_space(1, x, columns, sx, prec);
+    check(launch_column_scaled_add(sx, long(n_), y, long(n_), long(n_), columns, theta, -1.0, ctx_.stream()),
+          "shift");
+  } else {
+    for (long c0 = 0; c0 < columns; c0 += long(chunk_)) {
+      const long c = std::min<long>(long(chunk_), columns - c0);
+      const long off = 2 * c0 * long(n_);
+      apply_chunk(2, x + off, c, y + off, nullptr, prec, owned_staging(2, c, prec), theta + c0);
+    }
+  }
+  if (ctx_.take_fault(Context::kFaultNonfiniteApply)) {
+    // injected fault: all-ones bits are a NaN double
+    check(hipMemsetAsync(y, 0xFF, sizeof(double), ctx_.stream()), "fault injection");
+  }
+}
+
 namespace {
 std::size_t align256(std::size_t b) { return (b + 255) / 256 * 256; }
 }  // namespace
@@ -1144,7 +1167,7 @@ void DeviceOperator::projector_metric(const double* p, long columns, double* qp)
 // P = B X once for the pair, onsite D/Q blocks, then B^H (.) added with the
 // kinetic/local H or the (1+reg) identity of S folded into the GEMM epilogue.
 void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy, double* sy, Precision prec,
-                                 const Staging& st) {
+                                 const Staging& st, const double* shift) {
   hipStream_t s = ctx_.stream();
   const long n = long(n_), np = long(np_);
   const bool want_h = which != 1, want_s = which != 0;
@@ -1204,6 +1227,16 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
             "back projection");
     }
   };
+  if (shift) {
+    // (H - theta S) x = local H x + B^H (D - theta Q) P - theta (1 + reg) x
+    if (np) {
+      check(launch_column_scaled_add(st.w + 2 * np * c, np, st.w, np, np, c, shift, -1.0, s), "onsite shift");
+      if (overlap) ctx_.join();
+      backproject(st.w, hy, hy, 1.0);
+    }
+    check(launch_column_scaled_add(x, n, hy, n, n, c, shift, -(1.0 + reg_), s), "identity shift");
+    return;
+  }
   if (np) {
     if (want_s) backproject(st.w + 2 * np * c, sy, x, 1.0 + reg_);
     if (want_h) {

@@ -258,6 +258,10 @@ class DeviceOperator {
   void apply_noalloc(int which, const double* x, long columns, double* hy, double* sy, Precision prec);
   // scratch bytes apply_noalloc needs for one pass of `columns` columns
   std::size_t noalloc_bytes(int which, long columns, Precision prec) const;
+  // y(:, j) = (H - theta[j] S) x(:, j): one back-projection of the shifted
+  // onsite products (D - theta_j Q) P instead of one each for H and S (the
+  // correction equations' operator); theta: device array of `columns` shifts.
+  void apply_shifted(const double* x, long columns, const double* theta, double* y, Precision prec);
   // P = B X (np x columns, ld np) and QP = Q P with the onsite metric
   // (src/plane_wave_operator.cpp:375-389); device pointers.
   void project(const double* x, long columns, double* p, Precision prec);
@@ -340,8 +344,9 @@ class DeviceOperator {
   };
   StagingBytes staging_bytes(int which, long columns, Precision prec) const;
   Staging owned_staging(int which, long columns, Precision prec);
+  // shift != nullptr (which 2, sy unused): hy = (H - shift_j S) x_j
   void apply_chunk(int which, const double* x, long columns, double* hy, double* sy, Precision prec,
-                   const Staging& st);
+                   const Staging& st, const double* shift = nullptr);
   std::shared_ptr<void> davidson_cache_;
   Context& ctx_;
   std::size_t n_ = 0, np_ = 0, chunk_ = 256;
@@ -361,6 +366,7 @@ class DeviceOperator {
   int atoms_ = 0;
   DevBuf<char> t1_, t2_;
   DevBuf<double> p_, w_;
+  DevBuf<double> shift_work_;  // S x of a shifted application through the real-space search
   bool rs_enabled_ = false;
   RealSpaceDev rs_{};
   DevBuf<double> rs_w64_, rs_sdiag_;
