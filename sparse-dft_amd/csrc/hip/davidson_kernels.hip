@@ -9,6 +9,10 @@ namespace sdft_hip {
 
 namespace {
 constexpr int TPB = 256;
+// column reductions: one workgroup of RPB threads per column (the columns of a
+// block number a few hundred at most, so more threads per column is what puts
+// enough loads in flight), two entries per thread and trip
+constexpr int RPB = 1024;
 
 // error-free transformation a + b = s + e
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
@@ -26,12 +30,13 @@ __device__ __forceinline__ void dd_add(double& hi, double& lo, double h2, double
   two_sum(s, e, hi, lo);
 }
 
+template <int NT = TPB>
 __device__ __forceinline__ void block_reduce_dd(double& hi, double& lo, double* sh, double* sl) {
   const int t = threadIdx.x;
   sh[t] = hi;
   sl[t] = lo;
   __syncthreads();
-  for (int s = TPB / 2; s > 0; s >>= 1) {
+  for (int s = NT / 2; s > 0; s >>= 1) {
     if (t < s) {
       double h = sh[t], l = sl[t];
       dd_add(h, l, sh[t + s], sl[t + s]);
@@ -46,22 +51,31 @@ __device__ __forceinline__ void block_reduce_dd(double& hi, double& lo, double* 
 
 // one block per column: sum_i re(a_i) re(b_i) + im(a_i) im(b_i), compensated
 // (FMA-exact products accumulated in double-double; fixed reduction order)
-__global__ __launch_bounds__(TPB) void k_real_dots(const double* __restrict__ a, long lda,
+__global__ __launch_bounds__(RPB) void k_real_dots(const double* __restrict__ a, long lda,
                                                    const double* __restrict__ b, long ldb, long n,
                                                    double* __restrict__ out) {
 #pragma clang fp contract(off)
-  __shared__ double sh[TPB], sl[TPB];
+  __shared__ double sh[RPB], sl[RPB];
   const long col = blockIdx.x;
-  const double* ac = a + 2 * col * lda;
-  const double* bc = b + 2 * col * ldb;
+  const double2* ac = reinterpret_cast<const double2*>(a + 2 * col * lda);
+  const double2* bc = reinterpret_cast<const double2*>(b + 2 * col * ldb);
   double hi = 0, lo = 0;
-  for (long i = threadIdx.x; i < n; i += TPB) {
-    const double ar = ac[2 * i], ai = ac[2 * i + 1], br = bc[2 * i], bi = bc[2 * i + 1];
-    const double p1 = ar * br, p2 = ai * bi;
-    dd_add(hi, lo, p1, __builtin_fma(ar, br, -p1));
-    dd_add(hi, lo, p2, __builtin_fma(ai, bi, -p2));
+  long i = threadIdx.x;
+  for (; i + RPB < n; i += 2 * RPB) {
+    const double2 x0 = ac[i], y0 = bc[i], x1 = ac[i + RPB], y1 = bc[i + RPB];
+    const double p1 = x0.x * y0.x, p2 = x0.y * y0.y, p3 = x1.x * y1.x, p4 = x1.y * y1.y;
+    dd_add(hi, lo, p1, __builtin_fma(x0.x, y0.x, -p1));
+    dd_add(hi, lo, p2, __builtin_fma(x0.y, y0.y, -p2));
+    dd_add(hi, lo, p3, __builtin_fma(x1.x, y1.x, -p3));
+    dd_add(hi, lo, p4, __builtin_fma(x1.y, y1.y, -p4));
   }
-  block_reduce_dd(hi, lo, sh, sl);
+  if (i < n) {
+    const double2 x0 = ac[i], y0 = bc[i];
+    const double p1 = x0.x * y0.x, p2 = x0.y * y0.y;
+    dd_add(hi, lo, p1, __builtin_fma(x0.x, y0.x, -p1));
+    dd_add(hi, lo, p2, __builtin_fma(x0.y, y0.y, -p2));
+  }
+  block_reduce_dd<RPB>(hi, lo, sh, sl);
   if (threadIdx.x == 0) out[col] = hi + lo;
 }
 
@@ -95,16 +109,23 @@ __global__ __launch_bounds__(TPB) void k_complex_dots(const double* __restrict__
 }
 
 // per column: sqrt(sum |x|^2)
-__global__ __launch_bounds__(TPB) void k_col_norms(const double* __restrict__ x, long ldx, long n,
+__global__ __launch_bounds__(RPB) void k_col_norms(const double* __restrict__ x, long ldx, long n,
                                                    double* __restrict__ out) {
-  __shared__ double sh[TPB];
+  __shared__ double sh[RPB];
   const long col = blockIdx.x;
-  const double* xc = x + 2 * col * ldx;
-  double s = 0;
-  for (long i = threadIdx.x; i < n; i += TPB) s += xc[2 * i] * xc[2 * i] + xc[2 * i + 1] * xc[2 * i + 1];
+  const double2* xc = reinterpret_cast<const double2*>(x + 2 * col * ldx);
+  double s = 0, s1 = 0;
+  long i = threadIdx.x;
+  for (; i + RPB < n; i += 2 * RPB) {
+    const double2 v0 = xc[i], v1 = xc[i + RPB];
+    s += v0.x * v0.x + v0.y * v0.y;
+    s1 += v1.x * v1.x + v1.y * v1.y;
+  }
+  if (i < n) s += xc[i].x * xc[i].x + xc[i].y * xc[i].y;
+  s += s1;
   sh[threadIdx.x] = s;
   __syncthreads();
-  for (int k = TPB / 2; k > 0; k >>= 1) {
+  for (int k = RPB / 2; k > 0; k >>= 1) {
     if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
     __syncthreads();
   }
@@ -112,17 +133,24 @@ __global__ __launch_bounds__(TPB) void k_col_norms(const double* __restrict__ x,
 }
 
 // per column: sum_i |x_i|^2 T_i
-__global__ __launch_bounds__(TPB) void k_kinetic_weights(const double* __restrict__ x, long ldx, long n,
+__global__ __launch_bounds__(RPB) void k_kinetic_weights(const double* __restrict__ x, long ldx, long n,
                                                          const double* __restrict__ kinetic,
                                                          double* __restrict__ out) {
-  __shared__ double sh[TPB];
+  __shared__ double sh[RPB];
   const long col = blockIdx.x;
-  const double* xc = x + 2 * col * ldx;
-  double s = 0;
-  for (long i = threadIdx.x; i < n; i += TPB) s += (xc[2 * i] * xc[2 * i] + xc[2 * i + 1] * xc[2 * i + 1]) * kinetic[i];
+  const double2* xc = reinterpret_cast<const double2*>(x + 2 * col * ldx);
+  double s = 0, s1 = 0;
+  long i = threadIdx.x;
+  for (; i + RPB < n; i += 2 * RPB) {
+    const double2 v0 = xc[i], v1 = xc[i + RPB];
+    s += (v0.x * v0.x + v0.y * v0.y) * kinetic[i];
+    s1 += (v1.x * v1.x + v1.y * v1.y) * kinetic[i + RPB];
+  }
+  if (i < n) s += (xc[i].x * xc[i].x + xc[i].y * xc[i].y) * kinetic[i];
+  s += s1;
   sh[threadIdx.x] = s;
   __syncthreads();
-  for (int k = TPB / 2; k > 0; k >>= 1) {
+  for (int k = RPB / 2; k > 0; k >>= 1) {
     if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
     __syncthreads();
   }
@@ -458,7 +486,7 @@ __global__ __launch_bounds__(TPB) void k_davidson_seeds(long n, long warm_cols, 
 hipError_t launch_real_dots(const double* a, long lda, const double* b, long ldb, long n, long cols, double* out,
                             hipStream_t s) {
   if (cols <= 0) return hipSuccess;
-  SDFT_CHECKED(hipLaunchKernelGGL(k_real_dots, dim3(unsigned(cols)), dim3(TPB), 0, s, a, lda, b, ldb, n, out));
+  SDFT_CHECKED(hipLaunchKernelGGL(k_real_dots, dim3(unsigned(cols)), dim3(RPB), 0, s, a, lda, b, ldb, n, out));
 }
 hipError_t launch_complex_dots(const double* a, long lda, long ca, const double* b, long ldb, long cb, long n,
                                double* out, long ldo, hipStream_t s) {
@@ -468,7 +496,7 @@ hipError_t launch_complex_dots(const double* a, long lda, long ca, const double*
 }
 hipError_t launch_col_norms(const double* x, long ldx, long n, long cols, double* out, hipStream_t s) {
   if (cols <= 0) return hipSuccess;
-  SDFT_CHECKED(hipLaunchKernelGGL(k_col_norms, dim3(unsigned(cols)), dim3(TPB), 0, s, x, ldx, n, out));
+  SDFT_CHECKED(hipLaunchKernelGGL(k_col_norms, dim3(unsigned(cols)), dim3(RPB), 0, s, x, ldx, n, out));
 }
 hipError_t launch_davidson_seeds(long n, long warm_cols, const double* init, long ldi, double pert, long guards,
                                  const int* order, double* out, long ldo, hipStream_t s) {
@@ -479,7 +507,7 @@ hipError_t launch_davidson_seeds(long n, long warm_cols, const double* init, lon
 hipError_t launch_kinetic_weights(const double* x, long ldx, long n, long cols, const double* kinetic, double* out,
                                   hipStream_t s) {
   if (cols <= 0) return hipSuccess;
-  SDFT_CHECKED(hipLaunchKernelGGL(k_kinetic_weights, dim3(unsigned(cols)), dim3(TPB), 0, s, x, ldx, n, kinetic, out));
+  SDFT_CHECKED(hipLaunchKernelGGL(k_kinetic_weights, dim3(unsigned(cols)), dim3(RPB), 0, s, x, ldx, n, kinetic, out));
 }
 hipError_t launch_precondition(const double* r, long ldr, const int* src, long cols, const double* wk,
                                const double* kinetic, long n, double* out, long ldo, hipStream_t s) {

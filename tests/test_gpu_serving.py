@@ -51,13 +51,29 @@ def test_mixed_requests_match_solo_solves():
         assert np.array_equal(alone["eigenvectors"], r["eigenvectors"])
 
 
+def _same_solution(alone, r):
+    """Bitwise equal when neither solve's Rayleigh-Ritz fell back to zheevd
+    (a reduction whose bounded wait expired while other workers held the
+    CUs, which rounds differently); otherwise equal to the certificate's
+    accuracy."""
+    fallbacks = (alone["rr_barrier_fallbacks"], r["rr_barrier_fallbacks"])
+    if fallbacks == (0, 0):
+        assert np.array_equal(alone["eigenvalues"], r["eigenvalues"])
+        assert np.array_equal(alone["eigenvectors"], r["eigenvectors"])
+    else:
+        assert np.abs(alone["eigenvalues"] - r["eigenvalues"]).max() < 1e-10, fallbacks
+        x, y = alone["eigenvectors"], r["eigenvectors"]
+        assert np.linalg.norm(x @ x.conj().T - y @ y.conj().T, 2) < 1e-6, fallbacks
+
+
 DAVIDSON = dict(search_precision=1, refinement=1, search_handoff=3e-3)
 
 
 def test_serving_engine_mix():
     """Config-4 serving: Si8/Si16 requests at mixed cutoffs submitted at once
     to the serving engine (worker processes, one HIP context each) with an HBM
-    budget that admits only part of them at a time. Every certified result equals its solo solve bitwise; the
+    budget that admits only part of them at a time. Every certified result equals its solo solve
+    bitwise (to the certificate's accuracy after a Rayleigh-Ritz fallback); the
     request with an injected nonfinite application and the one with an
     over-budget estimate fail alone; the in-flight estimate never exceeds
     the budget while several requests ran concurrently."""
@@ -87,5 +103,4 @@ def test_serving_engine_mix():
             op.close()
         finally:
             ctx.close()
-        assert np.array_equal(alone["eigenvalues"], r["eigenvalues"])
-        assert np.array_equal(alone["eigenvectors"], r["eigenvectors"])
+        _same_solution(alone, r)
