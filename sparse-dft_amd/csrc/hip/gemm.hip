@@ -273,8 +273,21 @@ __global__ void k_splitk_reduce(int M, int N, int splits, const double* __restri
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
     const int m = int(e % M);
     const long n = e / M;
+    // partials summed in split order (deterministic); eight loads issued
+    // ahead of their additions
     double re = 0, im = 0;
-    for (int s = 0; s < splits; ++s) {
+    int s = 0;
+    for (; s + 8 <= splits; s += 8) {
+      double2 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const double2*>(work + 2 * ((long)(s + u) * total + e));
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        re += v[u].x;
+        im += v[u].y;
+      }
+    }
+    for (; s < splits; ++s) {
       const double* w = work + 2 * ((long)s * total + e);
       re += w[0];
       im += w[1];
