@@ -1,8 +1,8 @@
 // Complex GEMM on CDNA4 matrix cores, column-major with explicit leading
 // dimensions (the semantics of src/compute_backend.hpp:21-27):
 //   C = op(A) op(B) + beta * D,  op in {Normal, Adjoint},  D may alias C.
-// Complex products use four real MFMAs per 16x16x4 block:
-//   re += Ar Br - Ai Bi,  im += Ar Bi + Ai Br.
+// Complex products use three real MFMAs per 16x16x4 block (Gauss form, below):
+//   P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); re = P1 - P2, im = P3 - P1 - P2.
 // FP64 uses v_mfma_f64_16x16x4_f64, FP32 uses v_mfma_f32_16x16x4_f32 (inputs
 // stored in FP64 or FP32 are converted while staging into LDS).
 // Long contractions (K >> M, N: projections B X, Galerkin slabs) are split
@@ -69,11 +69,15 @@ __device__ __forceinline__ TileId xcd_tile(int mt, int nt, int splits) {
   return {(t / nt) % mt, t % nt, t / (nt * mt)};
 }
 
-// FP64 complex products as three real MFMA products (Gauss / 3M: Ar Br, Ai Bi,
-// (Ar + Ai)(Br + Bi)) instead of four, 25 % fewer MFMA cycles; FP32 keeps four.
-// SDFT_GEMM_GAUSS_FP64=0 builds the four-product form.
+// Complex products as three real MFMA products (Gauss / 3M: Ar Br, Ai Bi,
+// (Ar + Ai)(Br + Bi)) instead of four, 25 % fewer MFMA cycles, in both
+// precisions; SDFT_GEMM_GAUSS_FP64=0 / SDFT_GEMM_GAUSS_FP32=0 build the
+// four-product form.
 #ifndef SDFT_GEMM_GAUSS_FP64
 #define SDFT_GEMM_GAUSS_FP64 1
+#endif
+#ifndef SDFT_GEMM_GAUSS_FP32
+#define SDFT_GEMM_GAUSS_FP32 1
 #endif
 
 // WAVES: at least that many waves per SIMD. Left alone, the FP64 tiles keep
@@ -105,9 +109,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
   const int m0 = tile.m * TBM, n0 = tile.n * TBN;
   const int k0 = tile.split * kps, k1 = min(K, k0 + kps);
   const int wm = (wave % WGM) * WM, wn = (wave / WGM) * WN;
-  // FP64 with kGauss: three real products per block (Gauss / 3M),
+  // kGauss: three real products per block (Gauss / 3M),
   // acc_re = sum Ar Br, acc_x = sum Ai Bi, acc_im = sum (Ar + Ai)(Br + Bi)
-  constexpr bool kGauss = sizeof(T) == 8 && SDFT_GEMM_GAUSS_FP64;
+  constexpr bool kGauss = sizeof(T) == 8 ? SDFT_GEMM_GAUSS_FP64 : SDFT_GEMM_GAUSS_FP32;
   constexpr int GI = kGauss ? IM : 1, GJ = kGauss ? JN : 1;
   V acc_re[IM][JN], acc_im[IM][JN], acc_x[GI][GJ];
 #pragma unroll
@@ -472,7 +476,9 @@ Plan plan_for(bool fp64, int aop, int M, int N, int K, bool have_work) {
 }
 }  // namespace
 
-int zgemm_real_products(Precision prec) { return prec == Precision::FP64 && SDFT_GEMM_GAUSS_FP64 ? 3 : 4; }
+int zgemm_real_products(Precision prec) {
+  return (prec == Precision::FP64 ? SDFT_GEMM_GAUSS_FP64 : SDFT_GEMM_GAUSS_FP32) ? 3 : 4;
+}
 
 std::size_t zgemm_workspace_bytes(int M, int N, int K) {
   const int splits = zgemm_splits(M, N, K);
