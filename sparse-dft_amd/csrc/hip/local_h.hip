@@ -35,6 +35,12 @@ __device__ __forceinline__ cx<T> load_cx(const double* p) {
 
 }  // namespace
 
+// Elements of a stick workgroup per thread whose loads are issued together
+// (sticks_per_block keeps a workgroup at <= 2048 elements, 8 per thread of
+// 256): the sphere indices first, then every gather they address, so a thread
+// waits for two memory round trips instead of two per element.
+constexpr int kGatherPer = 8;
+
 // A: grid (ceil(nsticks / spb), columns)
 template <typename T>
 __global__ __launch_bounds__(256) void k_sticks_backward(const double* __restrict__ x, long ldx,
@@ -48,10 +54,26 @@ __global__ __launch_bounds__(256) void k_sticks_backward(const double* __restric
   const long col = blockIdx.y;
   const double* xc = x + 2 * col * ldx;
   const int ls = line_stride(n2);
-  for (int e = tid; e < nl * n2; e += nt) {
-    const int s = e / n2, z = e - s * n2;
-    const int g = stick_g[(long)(first + s) * n2 + z];
-    lines[s * ls + z] = g >= 0 ? load_cx<T>(xc + 2L * g) : cx<T>{T(0), T(0)};
+  const int total = nl * n2;
+  const int* sg = stick_g + (long)first * n2;
+  for (int e0 = 0; e0 < total; e0 += kGatherPer * nt) {
+    int g[kGatherPer];
+#pragma unroll
+    for (int u = 0; u < kGatherPer; ++u) {
+      const int e = e0 + tid + u * nt;
+      g[u] = e < total ? sg[e] : -1;
+    }
+    cx<T> v[kGatherPer];
+#pragma unroll
+    for (int u = 0; u < kGatherPer; ++u) v[u] = g[u] >= 0 ? load_cx<T>(xc + 2L * g[u]) : cx<T>{T(0), T(0)};
+#pragma unroll
+    for (int u = 0; u < kGatherPer; ++u) {
+      const int e = e0 + tid + u * nt;
+      if (e < total) {
+        const int s = e / n2, z = e - s * n2;
+        lines[s * ls + z] = v[u];
+      }
+    }
   }
   __syncthreads();
   LineGeom geom{nl, ls, 1, false};
@@ -195,17 +217,35 @@ __global__ __launch_bounds__(256) void k_sticks_forward(const cx<T>* __restrict_
   line_fft_dispatch<T, -1, false>(n2, lines, geom, tw_z, tid, nt);
   const double* xc = x + 2 * col * ldx;
   double* yc = y + 2 * col * ldy;
-  for (int e = tid; e < nl * n2; e += nt) {
-    const int s = e / n2, z = e - s * n2;
-    const int g = stick_g[(long)(first + s) * n2 + z];
-    if (g < 0) continue;
-    const double t = kinetic[g];
-    const double2 xi = *reinterpret_cast<const double2*>(xc + 2L * g);
-    const cx<T> w = lines[s * ls + z];
-    double2 r;
-    r.x = t * xi.x + double(w.x) / count;
-    r.y = t * xi.y + double(w.y) / count;
-    *reinterpret_cast<double2*>(yc + 2L * g) = r;
+  const int total = nl * n2;
+  const int* sg = stick_g + (long)first * n2;
+  for (int e0 = 0; e0 < total; e0 += kGatherPer * nt) {
+    int g[kGatherPer];
+#pragma unroll
+    for (int u = 0; u < kGatherPer; ++u) {
+      const int e = e0 + tid + u * nt;
+      g[u] = e < total ? sg[e] : -1;
+    }
+    double t[kGatherPer];
+    double2 xi[kGatherPer];
+#pragma unroll
+    for (int u = 0; u < kGatherPer; ++u) {
+      if (g[u] >= 0) {
+        t[u] = kinetic[g[u]];
+        xi[u] = *reinterpret_cast<const double2*>(xc + 2L * g[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kGatherPer; ++u) {
+      if (g[u] < 0) continue;
+      const int e = e0 + tid + u * nt;
+      const int s = e / n2, z = e - s * n2;
+      const cx<T> w = lines[s * ls + z];
+      double2 r;
+      r.x = t[u] * xi[u].x + double(w.x) / count;
+      r.y = t[u] * xi[u].y + double(w.y) / count;
+      *reinterpret_cast<double2*>(yc + 2L * g[u]) = r;
+    }
   }
 }
 
