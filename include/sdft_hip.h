@@ -210,6 +210,11 @@ typedef struct {
   /* refinement: guard Ritz vectors carried beside the roots in its block
    * (1 .. roots; 0 = roots / 2) */
   long refinement_guards;
+  /* 1: initial and result->eigenvectors are device pointers (n x roots
+   * complex, resident in HBM; eigenvectors required), so that a caller whose
+   * blocks live on the device moves nothing over PCIe; 0: host pointers, as
+   * the reference's call takes them */
+  int device_pointers;
 } sdft_hip_davidson_params;
 
 typedef struct {

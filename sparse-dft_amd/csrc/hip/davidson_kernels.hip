@@ -257,6 +257,26 @@ __global__ void k_hermitise(const double* __restrict__ m, long ldm, long s, doub
   }
 }
 
+// Projected matrix C (leading dimension ld, order m) whose first k basis
+// columns are Ritz vectors of the previous Rayleigh-Ritz solve: their block is
+// diag(theta) (the identity when theta is null) and the rows below it are the
+// adjoint of the computed columns k..m, C(k:m, 0:k) = C(0:k, k:m)^H.
+__global__ void k_ritz_border(double* __restrict__ c, long ld, long k, long m, const double* __restrict__ theta) {
+  const long total = m * k;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const long i = e % m, j = e / m;
+    double re = 0., im = 0.;
+    if (i >= k) {
+      re = c[2 * (j + i * ld)];
+      im = -c[2 * (j + i * ld) + 1];
+    } else if (i == j) {
+      re = theta ? theta[i] : 1.;
+    }
+    c[2 * (i + j * ld)] = re;
+    c[2 * (i + j * ld) + 1] = im;
+  }
+}
+
 // U(:, j) *= 1 / sqrt(lambda_j)  (block CGS2 eigen-normalisation)
 __global__ void k_inv_sqrt_columns(double* __restrict__ u, long ld, long rows, const double* __restrict__ lambda) {
   const long col = blockIdx.y;
@@ -558,6 +578,12 @@ hipError_t launch_polish_update(const double* x, const double* d, long ld, long 
                                 double* trial, hipStream_t s) {
   if (cols <= 0) return hipSuccess;
   SDFT_CHECKED(hipLaunchKernelGGL(k_polish_update, grid_cols(n, cols), dim3(256), 0, s, x, d, ld, n, alpha, trial));
+}
+hipError_t launch_ritz_border(double* c, long ld, long k, long m, const double* theta, hipStream_t s) {
+  if (k <= 0 || m < k) return hipSuccess;
+  const long total = m * k;
+  const int blocks = int(total / 256 + 1 > 2048 ? 2048 : total / 256 + 1);
+  SDFT_CHECKED(hipLaunchKernelGGL(k_ritz_border, dim3(blocks), dim3(256), 0, s, c, ld, k, m, theta));
 }
 hipError_t launch_hermitise(const double* m, long ldm, long size, double* out, long ldo, hipStream_t s) {
   if (size <= 0) return hipSuccess;

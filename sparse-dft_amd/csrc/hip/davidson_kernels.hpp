@@ -30,6 +30,8 @@ hipError_t launch_divide_columns(double* x, long ld, long n, long cols, const do
 hipError_t launch_polish_update(const double* x, const double* d, long ld, long n, long cols, const double* alpha,
                                 double* trial, hipStream_t s);
 hipError_t launch_hermitise(const double* m, long ldm, long size, double* out, long ldo, hipStream_t s);
+// C(0:k, 0:k) = diag(theta) (identity for null theta), C(k:m, 0:k) = C(0:k, k:m)^H
+hipError_t launch_ritz_border(double* c, long ld, long k, long m, const double* theta, hipStream_t s);
 hipError_t launch_inv_sqrt_columns(double* u, long ld, long rows, long cols, const double* lambda, hipStream_t s);
 // Block preconditioned CG of the Jacobi-Davidson correction equations (one
 // CG per column, scalars in device arrays):

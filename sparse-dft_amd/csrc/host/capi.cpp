@@ -581,6 +581,9 @@ int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* par
     if (p.refinement_guards < 0 || p.refinement_guards > params->roots)
       throw std::invalid_argument("HIP: refinement guards must be in 0 .. roots");
     p.vectors_out = reinterpret_cast<std::complex<double>*>(result->eigenvectors);
+    if (params->device_pointers != 0 && params->device_pointers != 1)
+      throw std::invalid_argument("HIP: device_pointers must be 0 or 1");
+    p.device_pointers = params->device_pointers;
     if (p.search_fp32 != 0 && p.search_fp32 != 1) throw std::invalid_argument("HIP: search precision must be 0 or 1");
     const auto out = sdft_hip::davidson(op->impl, p, reinterpret_cast<const std::complex<double>*>(initial));
     std::copy(out.eigenvalues.begin(), out.eigenvalues.end(), result->eigenvalues);

@@ -792,6 +792,9 @@ bool Solver::refine(long& it, long& iterations) {
   int stalls = 0;
   double* wbuf = retained_.p;
   long rotated_at = -1;  // >= 0: W's rotated images wait in hb_/sb_ columns from there
+  // Ritz values of the previous step's Rayleigh-Ritz; empty while W still
+  // comes from the search subspace
+  std::vector<double> ritz;
   for (; it + 1 < p_.max_iterations;) {
     ++it;
     iterations = it + 1;
@@ -876,11 +879,26 @@ bool Solver::refine(long& it, long& iterations) {
     const long m = size_;
     double* A = rr_.p;
     double* B = rrb_.p;
-    gemm(true, 1, 0, m, m, n, 1.0, basis_.p, n, hb_.p, n, ch_.p, m);
-    gemm(true, 1, 0, m, m, n, 1.0, basis_.p, n, sb_.p, n, cs_.p, m);
+    // W holds Ritz vectors of the previous step's FP64 Rayleigh-Ritz, so its
+    // block of the pencil is (diag(theta), I) to rounding and only the columns
+    // of T are projected: m x |T| products instead of m x m (the first step's
+    // W comes from the search subspace and is projected in full)
+    const long first = ritz.empty() ? 0 : k;
+    gemm(true, 1, 0, m, m - first, n, 1.0, basis_.p, n, col(hb_, first, n), n, ch_.p + 2 * first * m, m);
+    gemm(true, 1, 0, m, m - first, n, 1.0, basis_.p, n, col(sb_, first, n), n, cs_.p + 2 * first * m, m);
+    if (first) {
+      // X was renormalised in step 1: its Rayleigh quotients, then the guards'
+      std::vector<double> theta(values_.begin(), values_.begin() + R);
+      theta.insert(theta.end(), ritz.begin() + R, ritz.begin() + k);
+      upload(small_.p + 8, theta.data(), k);
+      check(launch_ritz_border(ch_.p, m, k, m, small_.p + 8, s_), "Ritz block");
+      check(launch_ritz_border(cs_.p, m, k, m, nullptr, s_), "Ritz block");
+    }
     check(launch_hermitise(ch_.p, m, m, A, m, s_), "hermitise");
     check(launch_hermitise(cs_.p, m, m, B, m, s_), "hermitise");
     const std::vector<double> w = rr_solve(A, B, m, k);
+    if (long(w.size()) < k) throw std::runtime_error("HIP: refinement Rayleigh-Ritz returned too few values");
+    ritz.assign(w.begin(), w.begin() + k);
     for (long r = 0; r < R; ++r) values_[r] = w[r];
     gemm(true, 0, 0, n, k, m, 1.0, basis_.p, n, A, m, wbuf, n);
     // the next W's images, past the m columns in use when the buffers have room
@@ -930,7 +948,10 @@ DavidsonOutput Solver::run(const C* initial) {
   const std::vector<long> order = davidson_order(op_);
   const bool warm = initial != nullptr && 2 * R < n;
   const long nseeds = davidson_seed_columns(n, R, warm);
-  if (warm) ctx_.upload_bulk(corr_.p, initial, 16 * std::size_t(n) * R);
+  if (warm && p_.device_pointers)
+    check(hipMemcpyAsync(corr_.p, initial, 16 * std::size_t(n) * R, hipMemcpyDeviceToDevice, s_), "initial block");
+  else if (warm)
+    ctx_.upload_bulk(corr_.p, initial, 16 * std::size_t(n) * R);
   // the retained block holds cap_ columns; a local subspace of fewer than
   // 3 roots (band groups) takes a block of its own
   DevBuf<double> wide;
@@ -1132,7 +1153,10 @@ DavidsonOutput Solver::run(const C* initial) {
     out.eigenvectors.resize(std::size_t(n) * R);
     dst = out.eigenvectors.data();
   }
-  ctx_.download_bulk(dst, x_.p, 16 * std::size_t(n) * R);
+  if (p_.device_pointers)
+    check(hipMemcpyAsync(dst, x_.p, 16 * std::size_t(n) * R, hipMemcpyDeviceToDevice, s_), "eigenvectors");
+  else
+    ctx_.download_bulk(dst, x_.p, 16 * std::size_t(n) * R);
   sync();
   stats_.iterations = iterations;
   stats_.rr_stedc = eig_ ? eig_->stedc_solves() - stedc_base_ : 0;
@@ -1197,6 +1221,8 @@ StrictStepOutput strict_step(DeviceOperator& op, int kind, long roots, double to
 }
 
 DavidsonOutput davidson(DeviceOperator& op, const DavidsonParams& params, const std::complex<double>* initial) {
+  if (params.device_pointers && !params.vectors_out)
+    throw std::invalid_argument("HIP: a device-resident Davidson needs a device eigenvector destination");
   if (params.band_group_size > 0 && !params.band_group_local) return davidson_band_groups(op, params, initial);
   Solver s(op, params);
   return s.run(initial);
@@ -1225,8 +1251,10 @@ DavidsonOutput davidson_band_groups(DeviceOperator& op, const DavidsonParams& pa
   rot.reserve(std::size_t(2 * R * R));
   hh.reserve(std::size_t(2 * R * R));
   ss.reserve(std::size_t(2 * R * R));
+  const hipMemcpyKind in_kind = params.device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  const hipMemcpyKind out_kind = params.device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   if (initial) {
-    check(hipMemcpyAsync(x.p, initial, 16 * std::size_t(n) * R, hipMemcpyHostToDevice, s), "upload");
+    check(hipMemcpyAsync(x.p, initial, 16 * std::size_t(n) * R, in_kind, s), "upload");
   } else {
     check(hipMemsetAsync(x.p, 0, 16 * std::size_t(n) * R, s), "zero");
   }
@@ -1260,6 +1288,7 @@ DavidsonOutput davidson_band_groups(DeviceOperator& op, const DavidsonParams& pa
       lp.roots = count;
       lp.band_group_local = 1;
       lp.vectors_out = nullptr;
+      lp.device_pointers = 0;  // the group's seed and result pass through host vectors
       lp.capacity = std::min(n, 5 * G);
       if (first) {
         // S-orthonormalise the completed bands (Cholesky), src/band_group_solver.cpp:100-123
@@ -1360,7 +1389,7 @@ DavidsonOutput davidson_band_groups(DeviceOperator& op, const DavidsonParams& pa
     out.eigenvectors.resize(std::size_t(n) * R);
     dst = out.eigenvectors.data();
   }
-  check(hipMemcpyAsync(dst, x.p, 16 * std::size_t(n) * R, hipMemcpyDeviceToHost, s), "vectors");
+  check(hipMemcpyAsync(dst, x.p, 16 * std::size_t(n) * R, out_kind, s), "vectors");
   sync();
   out.stats.converged = converged;
   out.stats.seconds_total = now() - t_start;

@@ -41,6 +41,9 @@ struct DavidsonParams {
   // host n x roots destination of the eigenvectors: copied there directly
   // (DavidsonOutput::eigenvectors stays empty); nullptr keeps them in the output
   std::complex<double>* vectors_out = nullptr;
+  // 1: the initial block and vectors_out are device pointers (HBM resident,
+  // n x roots complex); vectors_out is then required
+  int device_pointers = 0;
 };
 
 struct DavidsonStats {

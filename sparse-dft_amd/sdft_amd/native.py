@@ -37,7 +37,7 @@ class DavidsonParams(ct.Structure):
                 ("subspace_factor", ct.c_long), ("search_precision", ct.c_int), ("search_handoff", ct.c_double),
                 ("stagnation_window", ct.c_int), ("handoff_restart", ct.c_int), ("verbose", ct.c_int),
                 ("timing", ct.c_int), ("band_group_size", ct.c_long), ("refinement", ct.c_int),
-                ("inner_iterations", ct.c_int), ("refinement_guards", ct.c_long)]
+                ("inner_iterations", ct.c_int), ("refinement_guards", ct.c_long), ("device_pointers", ct.c_int)]
 
 
 class DavidsonResult(ct.Structure):
@@ -160,6 +160,62 @@ def snapshot_info(path):
     info = SnapshotInfo()
     _check(lib().sdft_hip_snapshot_info_read(os.fsencode(path), ct.byref(info)))
     return {f: (tuple(getattr(info, f)) if f == "shape" else getattr(info, f)) for f, _ in SnapshotInfo._fields_}
+
+
+class DeviceBlock:
+    """A rows x cols complex128 column-major block in HBM, from the HIP runtime
+    libsdft_hip.so links (hipMalloc / hipMemcpy / hipFree), for callers that
+    keep their blocks on the device (`Operator.davidson(device_blocks=...)`,
+    `Operator.apply_device`)."""
+
+    _rt = None
+    _H2D, _D2H = 1, 2
+
+    @classmethod
+    def runtime(cls):
+        if cls._rt is None:
+            lib()
+            rt = ct.CDLL("libamdhip64.so.7")
+            rt.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
+            rt.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+            rt.hipFree.argtypes = [ct.c_void_p]
+            cls._rt = rt
+        return cls._rt
+
+    def __init__(self, rows, cols):
+        self.shape = (int(rows), int(cols))
+        self.nbytes = 16 * self.shape[0] * self.shape[1]
+        p = ct.c_void_p()
+        if self.runtime().hipMalloc(ct.byref(p), ct.c_size_t(self.nbytes)) != 0:
+            raise HipError(f"hipMalloc of {self.nbytes} bytes failed")
+        self.ptr = p.value
+
+    def upload(self, a):
+        a = np.asfortranarray(a, dtype=np.complex128)
+        if a.shape != self.shape:
+            raise ValueError(f"block shape {a.shape} != {self.shape}")
+        if self.runtime().hipMemcpy(ct.c_void_p(self.ptr), a.ctypes.data_as(ct.c_void_p), ct.c_size_t(a.nbytes),
+                                    self._H2D) != 0:
+            raise HipError("hipMemcpy to the device failed")
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, dtype=np.complex128, order="F")
+        if self.runtime().hipMemcpy(out.ctypes.data_as(ct.c_void_p), ct.c_void_p(self.ptr), ct.c_size_t(self.nbytes),
+                                    self._D2H) != 0:
+            raise HipError("hipMemcpy from the device failed")
+        return out
+
+    def close(self):
+        if self.ptr:
+            self.runtime().hipFree(ct.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Context:
@@ -558,19 +614,34 @@ class Operator:
     def davidson(self, roots, max_iterations=256, tolerance=1e-12, initial=None, subspace_factor=8,
                  search_precision=0, search_handoff=1e-8, stagnation_window=3, handoff_restart=1, verbose=0,
                  timing=0, want_vectors=True, band_group_size=0, refinement=0, inner_iterations=0,
-                 refinement_guards=0):
+                 refinement_guards=0, device_blocks=None):
+        """device_blocks = (initial, eigenvectors): device addresses of n x
+        roots complex128 column-major blocks resident in HBM (initial 0 for a
+        cold start); the solve then moves no block over PCIe and returns
+        eigenvectors=None (they are in the device block)."""
         p = DavidsonParams(roots, max_iterations, tolerance, subspace_factor, search_precision, search_handoff,
                            stagnation_window, handoff_restart, verbose, timing, band_group_size, refinement,
-                           inner_iterations, refinement_guards)
+                           inner_iterations, refinement_guards, 1 if device_blocks is not None else 0)
         ev = np.zeros(roots)
         res = np.zeros(roots)
-        vec = np.zeros((self.n, roots), dtype=np.complex128, order="F") if want_vectors else None
         r = DavidsonResult()
         r.eigenvalues = _p(ev)
-        r.eigenvectors = _p(vec) if vec is not None else None
         r.residual_norms = _p(res)
-        init = None if initial is None else np.asfortranarray(initial[:, :roots], dtype=np.complex128)
-        _check(lib().sdft_hip_davidson(self.handle, ct.byref(p), _p(init), ct.byref(r)))
+        if device_blocks is not None:
+            if initial is not None:
+                raise ValueError("device_blocks carries the initial block")
+            init_ptr, vec_ptr = device_blocks
+            if not vec_ptr:
+                raise ValueError("device_blocks needs a device eigenvector block")
+            vec = None
+            r.eigenvectors = ct.cast(ct.c_void_p(vec_ptr), _dp)
+            init = ct.cast(ct.c_void_p(init_ptr), _dp) if init_ptr else None
+        else:
+            vec = np.zeros((self.n, roots), dtype=np.complex128, order="F") if want_vectors else None
+            r.eigenvectors = _p(vec) if vec is not None else None
+            host_init = None if initial is None else np.asfortranarray(initial[:, :roots], dtype=np.complex128)
+            init = _p(host_init)
+        _check(lib().sdft_hip_davidson(self.handle, ct.byref(p), init, ct.byref(r)))
         return dict(eigenvalues=ev, eigenvectors=vec, residual_norms=res, iterations=r.iterations,
                     converged=bool(r.converged), h_columns_search=r.h_columns_search,
                     h_columns_exact=r.h_columns_exact, handoff_iteration=r.handoff_iteration,

@@ -86,6 +86,40 @@ def test_davidson_iteration_limited_reports_fresh_residuals(si8_dev, si8_dense):
     assert gerr < 1e-12
 
 
+@pytest.mark.parametrize("search_precision,band_group_size", [(0, 0), (1, 0), (0, 8)])
+def test_davidson_device_blocks_equal_host_buffers(si8, si8_dev, search_precision, band_group_size):
+    """device_pointers = 1 (initial block and eigenvectors resident in HBM)
+    runs the same kernels as the host-buffer call: the same eigenpairs bit
+    for bit, warm and cold, and the device block holds the eigenvectors."""
+    init = syn.initial_psi(si8, 16, 0)[:, :16]
+    kw = dict(search_precision=search_precision, refinement=search_precision, search_handoff=1.0,
+              band_group_size=band_group_size)
+    host = si8_dev.davidson(16, 256, 1e-12, init, **kw)
+    blocks = [native.DeviceBlock(si8.n, 16) for _ in range(2)]
+    try:
+        blocks[0].upload(init)
+        dev = si8_dev.davidson(16, 256, 1e-12, device_blocks=(blocks[0].ptr, blocks[1].ptr), **kw)
+        assert dev["eigenvectors"] is None and dev["converged"] and host["converged"]
+        assert np.array_equal(dev["eigenvalues"], host["eigenvalues"])
+        assert np.array_equal(blocks[1].download(), host["eigenvectors"])
+        assert np.array_equal(blocks[0].download(), np.asfortranarray(init))  # the initial block is only read
+        if not band_group_size:
+            cold_host = si8_dev.davidson(8, 256, 1e-12, search_precision=search_precision)
+            cold = si8_dev.davidson(8, 256, 1e-12, device_blocks=(0, blocks[1].ptr), search_precision=search_precision)
+            assert np.array_equal(cold["eigenvalues"], cold_host["eigenvalues"])
+            assert np.array_equal(blocks[1].download()[:, :8], cold_host["eigenvectors"])
+    finally:
+        for b in blocks:
+            b.close()
+
+
+def test_davidson_device_blocks_need_destination(si8_dev):
+    with pytest.raises(ValueError):
+        si8_dev.davidson(8, device_blocks=(0, 0))
+    with pytest.raises(ValueError):
+        si8_dev.davidson(8, initial=np.zeros((si8_dev.n, 8), complex), device_blocks=(0, 1))
+
+
 def test_davidson_invalid_arguments(si8_dev):
     with pytest.raises(ValueError):
         si8_dev.davidson(0)
