@@ -212,6 +212,10 @@ def main():
     ap.add_argument("--refinement-guards", type=int, default=60,
                     help="guard Ritz vectors beside the 160 roots in the refinement block (with 8 CG steps: "
                          "32 / 60 / 80 / 160 gave 746 / 780 / 766 / 711; 0 = roots / 2)")
+    ap.add_argument("--bf16-projectors", type=int, default=1,
+                    help="1: the refinement's correction solves run their projector GEMMs on BF16 MFMA with FP32 "
+                         "accumulation; 0: FP32 MFMA (1 / 0 gave 1056 / 1013 certified eigenpairs/s, 12 iterations "
+                         "either way)")
     ap.add_argument("--search-handoff", type=float, default=1.0,
                     help="max residual at which the FP32 search hands off (the reference hands off at 1e-8). With the "
                          "Jacobi-Davidson refinement taking its images by rotation, the earliest hand-off measured "
@@ -261,7 +265,7 @@ def main():
                         timing=args.timing, subspace_factor=args.subspace_factor,
                         stagnation_window=args.stagnation_window, handoff_restart=args.handoff_restart,
                         refinement=args.refinement if precision else 0, inner_iterations=args.inner_iterations,
-                        refinement_guards=args.refinement_guards,
+                        refinement_guards=args.refinement_guards, bf16_projectors=args.bf16_projectors,
                         search_handoff=args.search_handoff)
         ctx.synchronize()
         r["wall"] = time.perf_counter() - t0
@@ -295,9 +299,11 @@ def main():
     # Roofline of the kernel with the largest event-timed share of the timed steps.
     npw, count = prob.n, int(np.prod(prob.shape))
     peaks = {"gemm_fp64": (78.6, "TFLOP/s", "mfma", 1e12), "gemm_fp32": (157.3, "TFLOP/s", "mfma", 1e12),
+             "gemm_bf16": (2500.0, "TFLOP/s", "mfma", 1e12),
              "local_fp64": (PEAK_HBM_GBS, "GB/s", "hbm", 1e9), "local_fp32": (PEAK_HBM_GBS, "GB/s", "hbm", 1e9)}
     names = {"gemm_fp64": "sdft_hip::k_zgemm<double, ...> (FP64 MFMA complex GEMM, all shapes of the solve)",
              "gemm_fp32": "sdft_hip::k_zgemm<float, ...> (FP32 MFMA complex GEMM, all shapes of the solve)",
+             "gemm_bf16": "sdft_hip::k_zgemm_bf16 (BF16 MFMA projector GEMMs of the correction solves)",
              "local_fp64": "local-H pipeline FP64 (k_sticks_backward + k_planes + k_sticks_forward)",
              "local_fp32": "local-H pipeline FP32 (k_sticks_backward + k_planes + k_sticks_forward)"}
     kernels = {}
@@ -315,7 +321,8 @@ def main():
     traffic = measured_traffic().get(dom)
     # the GEMM kinds count 8 m n k flops; the FP64 kernels execute 2 x 3 m n k
     # real MFMA flops when they use the Gauss (3M) form
-    products = {"gemm_fp64": native.gemm_real_products(0), "gemm_fp32": native.gemm_real_products(1)}
+    products = {"gemm_fp64": native.gemm_real_products(0), "gemm_fp32": native.gemm_real_products(1),
+                "gemm_bf16": 4}
     for k in kernels:
         if k in products:
             kernels[k]["real_products_per_complex_mac"] = products[k]
@@ -433,6 +440,7 @@ def main():
                                                              refinement=args.refinement,
                                                              inner_iterations=args.inner_iterations,
                                                              refinement_guards=0,
+                                                             bf16_projectors=args.bf16_projectors,
                                                              search_handoff=args.search_handoff))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prob)

@@ -64,13 +64,14 @@ int sdft_hip_synchronize(sdft_hip_context* ctx);
  * HIP events are recorded on the context's stream around every MFMA complex
  * GEMM kernel and every local-Hamiltonian pipeline launch. Enabling or
  * disabling clears the sums. kind: 0 GEMM FP64, 1 GEMM FP32 arithmetic,
- * 2 local H FP64, 3 local H FP32. work: flops (8 m n k per GEMM) or the
+ * 2 local H FP64, 3 local H FP32, 4 GEMM BF16 arithmetic. work: flops (8 m n k per GEMM) or the
  * algorithmic bytes m (2 npw c + 4 N c) + N r + npw (4 + r) per local-H launch.
  * The stats call synchronises the stream. */
 #define SDFT_HIP_CLOCK_GEMM_FP64 0
 #define SDFT_HIP_CLOCK_GEMM_FP32 1
 #define SDFT_HIP_CLOCK_LOCAL_FP64 2
 #define SDFT_HIP_CLOCK_LOCAL_FP32 3
+#define SDFT_HIP_CLOCK_GEMM_BF16 4 /* BF16-MFMA projector GEMMs of the refinement's correction solves */
 int sdft_hip_kernel_clock(sdft_hip_context* ctx, int enable);
 int sdft_hip_kernel_clock_stats(sdft_hip_context* ctx, int kind, long* launches, double* milliseconds, double* work);
 /* Real MFMA products per complex multiply-add in the complex GEMM kernels of
@@ -155,6 +156,13 @@ int sdft_hip_operator_info(sdft_hip_operator* op, int shape[3], long* count, lon
  * device_pointers = 1: x, hy, sy are device pointers (n x columns). */
 int sdft_hip_operator_apply(sdft_hip_operator* op, int which, const double* x, size_t columns, double* hy, double* sy,
                             int precision, int device_pointers);
+/* y(:, j) = (H - theta_j S) x(:, j) with one back-projection of the shifted
+ * onsite products (no counterpart in the reference: the operator of the
+ * refinement's correction equations, exposed for tests). Host pointers, theta
+ * a host array of `columns` shifts; precision 0 FP64, 1 FP32, 2 FP32 with the
+ * projector GEMMs on BF16 MFMA (FP32 accumulation). */
+int sdft_hip_operator_apply_shifted(sdft_hip_operator* op, const double* x, size_t columns, const double* theta,
+                                    double* y, int precision);
 /* No counterpart in the reference. For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): sdft_hip_operator_apply with device_pointers = 1 that
@@ -215,6 +223,11 @@ typedef struct {
    * blocks live on the device moves nothing over PCIe; 0: host pointers, as
    * the reference's call takes them */
   int device_pointers;
+  /* refinement: 1 runs the projector GEMMs of the FP32 correction solves on
+   * BF16 MFMA with FP32 accumulation (the projectors as a resident BF16 copy;
+   * the correction equations are solved to a few digits only), 0 on FP32 MFMA.
+   * Rayleigh-Ritz, images and the certificate are unaffected. */
+  int bf16_projectors;
 } sdft_hip_davidson_params;
 
 typedef struct {

@@ -530,6 +530,210 @@ hipError_t zgemm_f32_storage(int aop, int bop, int M, int N, int K, double alpha
   return hipGetLastError();
 }
 
+// ---- BF16 projector products (search arithmetic) ----
+// C = alpha op(A) B + beta D with A the projector block stored as BF16 complex
+// pairs (real part in the low 16 bits), B and C / D stored in FP64, products on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation. Lane l of that instruction
+// holds 8 consecutive k of row (A) / column (B) l % 16, starting at
+// k = 8 (l / 16); its accumulators are laid out as the f32 16x16 ones
+// (register r: row 4 (l / 16) + r, column l % 16). Four real products per
+// complex block: at BF16 precision the sums of the three-product form would
+// lose the imaginary part to cancellation. One 128 x 32 tile per 256-thread
+// workgroup, 4 waves of 32 x 32 along M, K slabs of 32 staged in LDS as
+// [row][k] with a row stride of 40 elements (80 bytes, so the 8-element
+// operand reads are aligned 16-byte LDS reads), the next slab prefetched into
+// registers.
+namespace {
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef short s8v __attribute__((ext_vector_type(8)));
+constexpr int BKH = 32, SHB = BKH + 8, HTBM = 128, HTBN = 32;
+
+__device__ __forceinline__ __bf16 bf16_from_bits(unsigned short b) { return __builtin_bit_cast(__bf16, b); }
+__device__ __forceinline__ bf8 bf8_negate(bf8 v) {
+  s8v bits = __builtin_bit_cast(s8v, v);
+  bits ^= s8v{short(0x8000), short(0x8000), short(0x8000), short(0x8000), short(0x8000), short(0x8000),
+              short(0x8000), short(0x8000)};
+  return __builtin_bit_cast(bf8, bits);
+}
+
+template <bool ADJ_A>
+__global__ __launch_bounds__(256) void k_zgemm_bf16(int M, int N, int K, const unsigned* __restrict__ A, long lda,
+                                                    const double* __restrict__ B, long ldb, double* __restrict__ C,
+                                                    long ldc, const double* __restrict__ D, long ldd, double alpha,
+                                                    double beta, int kps, double* __restrict__ work) {
+  constexpr int WGM = 4, WM = HTBM / WGM, WN = HTBN, IM = WM / 16, JN = WN / 16;
+  constexpr int EA = HTBM * BKH / 256, EB = HTBN * BKH / 256;
+  __shared__ __attribute__((aligned(16))) __bf16 as_re[HTBM * SHB];
+  __shared__ __attribute__((aligned(16))) __bf16 as_im[HTBM * SHB];
+  __shared__ __attribute__((aligned(16))) __bf16 bs_re[HTBN * SHB];
+  __shared__ __attribute__((aligned(16))) __bf16 bs_im[HTBN * SHB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const TileId tile = xcd_tile(gridDim.x, gridDim.y, gridDim.z);
+  const int m0 = tile.m * HTBM, n0 = tile.n * HTBN;
+  const int k0 = tile.split * kps, k1 = min(K, k0 + kps);
+  const int wm = wave * WM;
+  f4 acc_re[IM][JN], acc_im[IM][JN];
+#pragma unroll
+  for (int i = 0; i < IM; ++i)
+#pragma unroll
+    for (int j = 0; j < JN; ++j) {
+      acc_re[i][j] = f4{0, 0, 0, 0};
+      acc_im[i][j] = f4{0, 0, 0, 0};
+    }
+  unsigned pa[EA];
+  double2 pb[EB];
+  auto load = [&](int kb) {
+#pragma unroll
+    for (int it = 0; it < EA; ++it) {
+      const int e = tid + it * 256;
+      int mm, kk;
+      if (!ADJ_A) { mm = e % HTBM; kk = e / HTBM; } else { kk = e % BKH; mm = e / BKH; }
+      const int m = m0 + mm, k = kb + kk;
+      pa[it] = (m < M && k < k1) ? (ADJ_A ? A[k + (long)m * lda] : A[m + (long)k * lda]) : 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < EB; ++it) {
+      const int e = tid + it * 256;
+      const int kk = e % BKH, nn = e / BKH;
+      const int n = n0 + nn, k = kb + kk;
+      pb[it] = (n < N && k < k1) ? *reinterpret_cast<const double2*>(B + 2 * (k + (long)n * ldb)) : double2{0, 0};
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int it = 0; it < EA; ++it) {
+      const int e = tid + it * 256;
+      int mm, kk;
+      if (!ADJ_A) { mm = e % HTBM; kk = e / HTBM; } else { kk = e % BKH; mm = e / BKH; }
+      // op(A) = A^H conjugates: flip the sign bit of the imaginary part
+      const unsigned short re = (unsigned short)(pa[it] & 0xffffu);
+      const unsigned short im = (unsigned short)((pa[it] >> 16) ^ (ADJ_A ? 0x8000u : 0u));
+      as_re[mm * SHB + kk] = bf16_from_bits(re);
+      as_im[mm * SHB + kk] = bf16_from_bits(im);
+    }
+#pragma unroll
+    for (int it = 0; it < EB; ++it) {
+      const int e = tid + it * 256;
+      const int kk = e % BKH, nn = e / BKH;
+      bs_re[nn * SHB + kk] = __bf16(float(pb[it].x));
+      bs_im[nn * SHB + kk] = __bf16(float(pb[it].y));
+    }
+  };
+  if (k0 < k1) load(k0);
+  for (int kb = k0; kb < k1; kb += BKH) {
+    store();
+    __syncthreads();
+    if (kb + BKH < k1) load(kb + BKH);
+    const int r = lane & 15, ko = 8 * (lane >> 4);
+    bf8 ar[IM], ai[IM], br[JN], bi[JN], nbi[JN];
+#pragma unroll
+    for (int i = 0; i < IM; ++i) {
+      ar[i] = *reinterpret_cast<const bf8*>(&as_re[(wm + 16 * i + r) * SHB + ko]);
+      ai[i] = *reinterpret_cast<const bf8*>(&as_im[(wm + 16 * i + r) * SHB + ko]);
+    }
+#pragma unroll
+    for (int j = 0; j < JN; ++j) {
+      br[j] = *reinterpret_cast<const bf8*>(&bs_re[(16 * j + r) * SHB + ko]);
+      bi[j] = *reinterpret_cast<const bf8*>(&bs_im[(16 * j + r) * SHB + ko]);
+      nbi[j] = bf8_negate(bi[j]);
+    }
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+      for (int j = 0; j < JN; ++j) {
+        acc_re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[i], br[j], acc_re[i][j], 0, 0, 0);
+        acc_im[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[i], bi[j], acc_im[i][j], 0, 0, 0);
+      }
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+      for (int j = 0; j < JN; ++j) {
+        acc_re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai[i], nbi[j], acc_re[i][j], 0, 0, 0);
+        acc_im[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai[i], br[j], acc_im[i][j], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+  const bool split = work != nullptr;
+#pragma unroll
+  for (int i = 0; i < IM; ++i)
+#pragma unroll
+    for (int j = 0; j < JN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        const int n = n0 + 16 * j + (lane & 15);
+        if (m >= M || n >= N) continue;
+        double re = alpha * double(acc_re[i][j][r]), im = alpha * double(acc_im[i][j][r]);
+        if (split) {
+          double* w = work + 2 * ((long)tile.split * M * N + m + (long)n * M);
+          w[0] = re;
+          w[1] = im;
+        } else {
+          if (beta != 0.0) {
+            const double* d = D + 2 * (m + (long)n * ldd);
+            re += beta * d[0];
+            im += beta * d[1];
+          }
+          double* c = C + 2 * (m + (long)n * ldc);
+          c[0] = re;
+          c[1] = im;
+        }
+      }
+}
+
+// FP32 / FP64 complex pairs -> BF16 pairs (round to nearest even)
+template <typename S>
+__global__ void k_to_bf16_pairs(const S* __restrict__ x, long count, unsigned* __restrict__ out) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < count; e += (long)gridDim.x * blockDim.x) {
+    const unsigned short re = __builtin_bit_cast(unsigned short, __bf16(float(x[2 * e])));
+    const unsigned short im = __builtin_bit_cast(unsigned short, __bf16(float(x[2 * e + 1])));
+    out[e] = unsigned(re) | (unsigned(im) << 16);
+  }
+}
+}  // namespace
+
+hipError_t zgemm_bf16_storage(int aop, int M, int N, int K, double alpha, const unsigned* A, long lda,
+                              const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta,
+                              double* work, std::size_t work_bytes, hipStream_t s, const LaunchEvents* ev) {
+  if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+  const long tiles = long((M + HTBM - 1) / HTBM) * ((N + HTBN - 1) / HTBN);
+  // long contractions split along K to >= 512 workgroups, within the workspace
+  int splits = 1;
+  while (tiles * splits < 512 && long(K) / (2 * splits) >= 256 && splits < 64) splits *= 2;
+  while (splits > 1 && (!work || std::size_t(splits) * M * N * 16 > work_bytes)) splits /= 2;
+  int kps = (K + splits - 1) / splits;
+  kps = ((kps + BKH - 1) / BKH) * BKH;
+  splits = (K + kps - 1) / kps;
+  const dim3 grid((M + HTBM - 1) / HTBM, (N + HTBN - 1) / HTBN, splits);
+  double* wk = splits > 1 ? work : nullptr;
+  if (ev && ev->begin) {
+    const hipError_t e = hipEventRecord(ev->begin, s);
+    if (e != hipSuccess) return e;
+  }
+  if (aop)
+    hipLaunchKernelGGL(k_zgemm_bf16<true>, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha,
+                       beta, kps, wk);
+  else
+    hipLaunchKernelGGL(k_zgemm_bf16<false>, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, alpha,
+                       beta, kps, wk);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && ev && ev->end) e = hipEventRecord(ev->end, s);
+  if (e != hipSuccess || splits == 1) return e;
+  hipLaunchKernelGGL(k_splitk_reduce, dim3(1024), dim3(256), 0, s, M, N, splits, work, C, ldc, D, ldd, beta);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_bf16_pairs(const float* x, long count, unsigned* out, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_to_bf16_pairs<float>, dim3(2048), dim3(256), 0, s, x, count, out);
+  return hipGetLastError();
+}
+hipError_t launch_to_bf16_pairs(const double* x, long count, unsigned* out, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_to_bf16_pairs<double>, dim3(2048), dim3(256), 0, s, x, count, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_atom_blocks(Precision prec, int np, int columns, int atoms, const int* atom_offset,
                               const int* atom_rows, const long* block_offset, const double* dblk, const double* qblk,
                               const double* P, long ldp, double* W, long ldw, int which_mask, hipStream_t s) {

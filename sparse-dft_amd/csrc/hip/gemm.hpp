@@ -31,6 +31,16 @@ hipError_t zgemm_f32_storage(int aop, int bop, int M, int N, int K, double alpha
                              const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta,
                              double* work, std::size_t work_bytes, hipStream_t s, const LaunchEvents* ev = nullptr);
 
+// C = alpha op(A) B + beta D (aop 0 = Normal, 1 = Adjoint) with A stored as BF16
+// complex pairs (real part in the low 16 bits of each 32-bit word), B, C, D
+// in FP64: BF16 MFMA products with FP32 accumulation (search arithmetic).
+hipError_t zgemm_bf16_storage(int aop, int M, int N, int K, double alpha, const unsigned* A, long lda,
+                              const double* B, long ldb, double* C, long ldc, const double* D, long ldd, double beta,
+                              double* work, std::size_t work_bytes, hipStream_t s, const LaunchEvents* ev = nullptr);
+// count complex values (FP32 or FP64 pairs) -> BF16 pairs, round to nearest even
+hipError_t launch_to_bf16_pairs(const float* x, long count, unsigned* out, hipStream_t s);
+hipError_t launch_to_bf16_pairs(const double* x, long count, unsigned* out, hipStream_t s);
+
 // W(:, 0:m) = D_a P_a and W(:, m:2m) = Q_a P_a per atom block (which_mask bit
 // 0: D, bit 1: Q). FP64 uses the compensated scalar order of the reference.
 hipError_t launch_atom_blocks(Precision prec, int np, int columns, int atoms, const int* atom_offset,

@@ -487,6 +487,24 @@ int sdft_hip_operator_apply(sdft_hip_operator* op, int which, const double* x, s
   });
 }
 
+int sdft_hip_operator_apply_shifted(sdft_hip_operator* op, const double* x, size_t columns, const double* theta,
+                                    double* y, int precision) {
+  return guarded([&] {
+    if (!op || !x || !theta || !y || !columns || precision < 0 || precision > 2)
+      throw std::invalid_argument("HIP: invalid shifted application arguments");
+    auto& o = op->impl;
+    hipStream_t s = o.context().stream();
+    const std::size_t total = 2 * o.n() * columns;
+    DevBuf<double> dx, dy, dt;
+    dx.upload(x, total, s);
+    dt.upload(theta, columns, s);
+    dy.reserve(total);
+    o.apply_shifted(dx.p, long(columns), dt.p, dy.p, precision ? Precision::FP32 : Precision::FP64, precision == 2);
+    check(hipMemcpyAsync(y, dy.p, total * 8, hipMemcpyDeviceToHost, s), "copy y");
+    o.context().sync();
+  });
+}
+
 int sdft_hip_operator_apply_noalloc(sdft_hip_operator* op, int which, const double* x, size_t columns, double* hy,
                                     double* sy, int precision) {
   return guarded([&] {
@@ -584,6 +602,9 @@ int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* par
     if (params->device_pointers != 0 && params->device_pointers != 1)
       throw std::invalid_argument("HIP: device_pointers must be 0 or 1");
     p.device_pointers = params->device_pointers;
+    if (params->bf16_projectors != 0 && params->bf16_projectors != 1)
+      throw std::invalid_argument("HIP: bf16_projectors must be 0 or 1");
+    p.bf16_projectors = params->bf16_projectors;
     if (p.search_fp32 != 0 && p.search_fp32 != 1) throw std::invalid_argument("HIP: search precision must be 0 or 1");
     const auto out = sdft_hip::davidson(op->impl, p, reinterpret_cast<const std::complex<double>*>(initial));
     std::copy(out.eigenvalues.begin(), out.eigenvalues.end(), result->eigenvalues);

@@ -755,7 +755,7 @@ void Solver::correction_cg(const std::vector<int>& active, double* t) {
   check(launch_real_dots(res, n, z, n, n, cols, rho0, s_), "dots");
   for (int it = 0; it < steps; ++it) {
     // q = (H - theta S) p in one application (a single back-projection)
-    op_.apply_shifted(p, cols, theta, q, f32);
+    op_.apply_shifted(p, cols, theta, q, f32, p_.bf16_projectors != 0);
     stats_.h_columns_search += cols;
     project(q, true);
     check(launch_real_dots(p, n, q, n, n, cols, pq, s_), "dots");

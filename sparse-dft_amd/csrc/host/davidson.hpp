@@ -44,6 +44,8 @@ struct DavidsonParams {
   // 1: the initial block and vectors_out are device pointers (HBM resident,
   // n x roots complex); vectors_out is then required
   int device_pointers = 0;
+  // refinement: projector GEMMs of the correction solves on BF16 MFMA (FP32 accumulation)
+  int bf16_projectors = 0;
 };
 
 struct DavidsonStats {

@@ -738,7 +738,17 @@ void DeviceOperator::apply(int which, const double* x, long columns, double* hy,
   }
 }
 
-void DeviceOperator::apply_shifted(const double* x, long columns, const double* theta, double* y, Precision prec) {
+const unsigned* DeviceOperator::projectors_bf16() {
+  if (!b16_.p) {
+    const std::size_t count = np_ * n_;
+    b16_.reserve(count);
+    check(launch_to_bf16_pairs(b32_.p, long(count), b16_.p, ctx_.stream()), "BF16 projectors");
+  }
+  return b16_.p;
+}
+
+void DeviceOperator::apply_shifted(const double* x, long columns, const double* theta, double* y, Precision prec,
+                                   bool bf16_projectors) {
   if (columns <= 0 || !x || !y || y == x || !theta)
     throw std::invalid_argument("HIP: invalid shifted plane-wave application buffers");
   if (rs_enabled_ && prec == Precision::FP32) {
@@ -752,7 +762,8 @@ void DeviceOperator::apply_shifted(const double* x, long columns, const double* 
     for (long c0 = 0; c0 < columns; c0 += long(chunk_)) {
       const long c = std::min<long>(long(chunk_), columns - c0);
       const long off = 2 * c0 * long(n_);
-      apply_chunk(2, x + off, c, y + off, nullptr, prec, owned_staging(2, c, prec), theta + c0);
+      apply_chunk(2, x + off, c, y + off, nullptr, prec, owned_staging(2, c, prec), theta + c0,
+                  bf16_projectors && prec != Precision::FP64);
     }
   }
   if (ctx_.take_fault(Context::kFaultNonfiniteApply)) {
@@ -1167,7 +1178,7 @@ void DeviceOperator::projector_metric(const double* p, long columns, double* qp)
 // P = B X once for the pair, onsite D/Q blocks, then B^H (.) added with the
 // kinetic/local H or the (1+reg) identity of S folded into the GEMM epilogue.
 void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy, double* sy, Precision prec,
-                                 const Staging& st, const double* shift) {
+                                 const Staging& st, const double* shift, bool bf16) {
   hipStream_t s = ctx_.stream();
   const long n = long(n_), np = long(np_);
   const bool want_h = which != 1, want_s = which != 0;
@@ -1204,7 +1215,13 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
   } else if (np) {
     if (prec == Precision::FP64)
       gemm(0, np, n, 1.0, b64_.p, np, x, n, st.p, np, nullptr, 0, 0.0);
-    else {
+    else if (bf16) {
+      const LaunchEvents ev =
+          ctx_.clock_launch(Context::kGemmBF16, 8.0 * np * c * n, gemm_algorithmic_bytes(np, c, n, 4, false));
+      check(zgemm_bf16_storage(0, int(np), int(c), int(n), 1.0, projectors_bf16(), np, x, n, st.p, np, nullptr, 0, 0.0,
+                               st.work, st.work_bytes, s, &ev),
+            "projection");
+    } else {
       const LaunchEvents ev =
           ctx_.clock_launch(Context::kGemmFP32, 8.0 * np * c * n, gemm_algorithmic_bytes(np, c, n, 8, false));
       check(zgemm_f32_storage(0, 0, int(np), int(c), int(n), 1.0, b32_.p, np, x, n, st.p, np, nullptr, 0, 0.0, st.work,
@@ -1219,7 +1236,13 @@ void DeviceOperator::apply_chunk(int which, const double* x, long c, double* hy,
   auto backproject = [&](const double* w, double* out, const double* d, double beta) {
     if (prec == Precision::FP64)
       gemm(1, n, np, 1.0, b64_.p, np, w, np, out, n, d, n, beta);
-    else {
+    else if (bf16) {
+      const LaunchEvents ev =
+          ctx_.clock_launch(Context::kGemmBF16, 8.0 * n * c * np, gemm_algorithmic_bytes(n, c, np, 4, beta != 0.0));
+      check(zgemm_bf16_storage(1, int(n), int(c), int(np), 1.0, projectors_bf16(), np, w, np, out, n, d, n, beta,
+                               st.work, st.work_bytes, s, &ev),
+            "back projection");
+    } else {
       const LaunchEvents ev =
           ctx_.clock_launch(Context::kGemmFP32, 8.0 * n * c * np, gemm_algorithmic_bytes(n, c, np, 8, beta != 0.0));
       check(zgemm_f32_storage(1, 0, int(n), int(c), int(np), 1.0, b32_.p, np, w, np, out, n, d, n, beta, st.work,

@@ -153,7 +153,7 @@ class Context {
 
   // Kernel clock: event pairs on this context's stream around every MFMA GEMM
   // kernel and every local-Hamiltonian pipeline launch, summed per kind.
-  enum ClockKind { kGemmFP64 = 0, kGemmFP32 = 1, kLocalFP64 = 2, kLocalFP32 = 3, kClockKinds = 4 };
+  enum ClockKind { kGemmFP64 = 0, kGemmFP32 = 1, kLocalFP64 = 2, kLocalFP32 = 3, kGemmBF16 = 4, kClockKinds = 5 };
   struct ClockStat {
     long launches = 0;
     double milliseconds = 0, work = 0;  // work: flops (GEMM) or algorithmic bytes (local H)
@@ -261,7 +261,10 @@ class DeviceOperator {
   // y(:, j) = (H - theta[j] S) x(:, j): one back-projection of the shifted
   // onsite products (D - theta_j Q) P instead of one each for H and S (the
   // correction equations' operator); theta: device array of `columns` shifts.
-  void apply_shifted(const double* x, long columns, const double* theta, double* y, Precision prec);
+  // bf16_projectors (FP32 only): the projection and back-projection GEMMs on
+  // BF16 MFMA with FP32 accumulation, the projectors as a resident BF16 copy.
+  void apply_shifted(const double* x, long columns, const double* theta, double* y, Precision prec,
+                     bool bf16_projectors = false);
   // P = B X (np x columns, ld np) and QP = Q P with the onsite metric
   // (src/plane_wave_operator.cpp:375-389); device pointers.
   void project(const double* x, long columns, double* p, Precision prec);
@@ -346,7 +349,9 @@ class DeviceOperator {
   Staging owned_staging(int which, long columns, Precision prec);
   // shift != nullptr (which 2, sy unused): hy = (H - shift_j S) x_j
   void apply_chunk(int which, const double* x, long columns, double* hy, double* sy, Precision prec,
-                   const Staging& st, const double* shift = nullptr);
+                   const Staging& st, const double* shift = nullptr, bool bf16_projectors = false);
+  // the projectors as BF16 complex pairs (np x n), made from the FP32 copy on first use
+  const unsigned* projectors_bf16();
   std::shared_ptr<void> davidson_cache_;
   Context& ctx_;
   std::size_t n_ = 0, np_ = 0, chunk_ = 256;
@@ -361,6 +366,7 @@ class DeviceOperator {
   DeviceGrid grid_;
   DevBuf<double> v64_, kinetic_d_, b64_, dblk_, qblk_;
   DevBuf<float> v32_, b32_;
+  DevBuf<unsigned> b16_;
   DevBuf<int> atom_offset_, atom_rows_;
   DevBuf<long> block_offset_;
   int atoms_ = 0;

@@ -37,7 +37,8 @@ class DavidsonParams(ct.Structure):
                 ("subspace_factor", ct.c_long), ("search_precision", ct.c_int), ("search_handoff", ct.c_double),
                 ("stagnation_window", ct.c_int), ("handoff_restart", ct.c_int), ("verbose", ct.c_int),
                 ("timing", ct.c_int), ("band_group_size", ct.c_long), ("refinement", ct.c_int),
-                ("inner_iterations", ct.c_int), ("refinement_guards", ct.c_long), ("device_pointers", ct.c_int)]
+                ("inner_iterations", ct.c_int), ("refinement_guards", ct.c_long), ("device_pointers", ct.c_int),
+                ("bf16_projectors", ct.c_int)]
 
 
 class DavidsonResult(ct.Structure):
@@ -283,7 +284,7 @@ class Context:
                                                 ct.c_size_t(k), _p(w), _p(v) if k else None, ct.byref(st)))
         return w, v, st.value
 
-    CLOCK_KINDS = {"gemm_fp64": 0, "gemm_fp32": 1, "local_fp64": 2, "local_fp32": 3}
+    CLOCK_KINDS = {"gemm_fp64": 0, "gemm_fp32": 1, "local_fp64": 2, "local_fp32": 3, "gemm_bf16": 4}
 
     def kernel_clock(self, enable):
         """Start (True) or stop (False) the per-launch HIP-event clock; both clear the sums."""
@@ -604,6 +605,18 @@ class Operator:
                                                        ct.c_size_t(columns), hy, sy, ct.c_int(precision))
         _check(rc)
 
+    def apply_shifted(self, x, theta, precision=0):
+        """(H - theta_j S) x_j per column (precision 0 FP64, 1 FP32, 2 FP32 with
+        BF16-MFMA projector GEMMs): the refinement's correction operator."""
+        x = np.asfortranarray(x, dtype=np.complex128)
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        if x.shape[0] != self.n or th.shape != (x.shape[1],):
+            raise ValueError("shifted application: x must be n x columns with one shift per column")
+        y = np.zeros_like(x)
+        _check(lib().sdft_hip_operator_apply_shifted(self.handle, _p(x), ct.c_size_t(x.shape[1]), _p(th), _p(y),
+                                                     ct.c_int(precision)))
+        return y
+
     def benchmark(self, columns, which="pair", precision=0, repetitions=10):
         ms = np.zeros(3)
         code = {"h": 0, "s": 1, "pair": 2}[which]
@@ -614,14 +627,15 @@ class Operator:
     def davidson(self, roots, max_iterations=256, tolerance=1e-12, initial=None, subspace_factor=8,
                  search_precision=0, search_handoff=1e-8, stagnation_window=3, handoff_restart=1, verbose=0,
                  timing=0, want_vectors=True, band_group_size=0, refinement=0, inner_iterations=0,
-                 refinement_guards=0, device_blocks=None):
+                 refinement_guards=0, device_blocks=None, bf16_projectors=0):
         """device_blocks = (initial, eigenvectors): device addresses of n x
         roots complex128 column-major blocks resident in HBM (initial 0 for a
         cold start); the solve then moves no block over PCIe and returns
         eigenvectors=None (they are in the device block)."""
         p = DavidsonParams(roots, max_iterations, tolerance, subspace_factor, search_precision, search_handoff,
                            stagnation_window, handoff_restart, verbose, timing, band_group_size, refinement,
-                           inner_iterations, refinement_guards, 1 if device_blocks is not None else 0)
+                           inner_iterations, refinement_guards, 1 if device_blocks is not None else 0,
+                           bf16_projectors)
         ev = np.zeros(roots)
         res = np.zeros(roots)
         r = DavidsonResult()

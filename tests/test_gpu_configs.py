@@ -66,7 +66,8 @@ def test_slab_davidson_against_reference(hip_ctx, name, kfrac):
 
 def test_si64_davidson_against_reference(hip_ctx):
     """The benchmark workload (Si64 Gamma, 160 roots, the bench's warm block):
-    the FP64 path and the FP32-search form (refinement on and off) certify on
+    the FP64 path and the FP32-search form (refinement on and off, its
+    correction solves with FP32 or BF16 projector products) certify on
     fresh FP64 applies and match the reference's compiled Davidson on the
     oracle operator within 1e-10 Ha for every eigenvalue."""
     prob = syn.config_problem(1)
@@ -82,7 +83,9 @@ def test_si64_davidson_against_reference(hip_ctx):
             dev.davidson(160, 256, 1e-12, init, subspace_factor=5, search_precision=1, refinement=1,
                          search_handoff=1.0, inner_iterations=6),
             dev.davidson(160, 256, 1e-12, init, subspace_factor=5, search_precision=1, refinement=1,
-                         search_handoff=1.0, inner_iterations=8, refinement_guards=60)]
+                         search_handoff=1.0, inner_iterations=8, refinement_guards=60),
+            dev.davidson(160, 256, 1e-12, init, subspace_factor=5, search_precision=1, refinement=1,
+                         search_handoff=1.0, inner_iterations=8, refinement_guards=60, bf16_projectors=1)]
     for r in runs:
         assert r["converged"]
         assert fresh_residuals(dev, r).max() <= 1.01e-12
