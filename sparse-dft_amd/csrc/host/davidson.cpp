@@ -507,15 +507,19 @@ bool Solver::cholesky_normalise(const double* v, const double* metric, const dou
   check(hipMemsetAsync(bits, 0, sizeof(unsigned long long), s_), "memset");
   check(launch_identity_defect(g2, cols, cols, nullptr, cols, bits, s_), "identity defect");
   const std::vector<double> h = to_host(small_.p, 8 + 2 * cols);
-  if (*reinterpret_cast<const int*>(h.data())) return false;
   double gmax = 0, umin = std::numeric_limits<double>::infinity();
   for (long i = 0; i < cols; ++i) {
     gmax = std::max(gmax, h[8 + i]);
     umin = std::min(umin, h[8 + cols + i] * h[8 + cols + i]);
   }
+  const double defect = h[2];  // the bits of a non-negative double (or NaN)
+  if (p_.verbose > 1)
+    std::fprintf(stderr, "  cholesky block of %ld: potrf info %d, trtri info %d, min pivot^2 / max diagonal %.3e, "
+                 "defect %.3e\n", cols, *reinterpret_cast<const int*>(h.data()),
+                 *reinterpret_cast<const int*>(h.data() + 1), umin / gmax, defect);
+  if (*reinterpret_cast<const int*>(h.data())) return false;
   if (!(gmax > 0) || !(umin >= 1e-8 * gmax) || !std::isfinite(gmax)) return false;
   if (*reinterpret_cast<const int*>(h.data() + 1)) return false;
-  const double defect = h[2];  // the bits of a non-negative double (or NaN)
   if (!(defect <= (exact ? 1e-10 : 1e-4))) return false;
   ++stats_.cholesky_blocks;
   return true;
@@ -848,6 +852,16 @@ bool Solver::refine(long& it, long& iterations) {
     // 2. corrections in search arithmetic
     double* t = w4_.p + 4 * n * R;
     correction_cg(active, t);
+    // the corrections scale with their roots' residuals, which span many
+    // orders of magnitude once some roots converge; unit columns keep the
+    // Cholesky pivot test below a test of linear dependence, not of scale
+    {
+      std::vector<double> norms = col_norms(t, n, cols);
+      for (double& v : norms)
+        if (!(v > 0) || !std::isfinite(v)) v = 1.0;
+      upload(small_.p + 8, norms.data(), cols);
+      check(launch_divide_columns(t, n, n, cols, small_.p + 8, s_), "unit corrections");
+    }
     // 3. T projected against W and S-orthonormalised, with FP64 images
     size_ = k;
     double* v = w3_.p;
@@ -855,13 +869,21 @@ bool Solver::refine(long& it, long& iterations) {
     double* coef = m1_.p;
     gemm(true, 1, 0, k, cols, n, 1.0, sb_.p, n, t, n, coef, k);
     gemm(true, 0, 0, n, cols, k, -1.0, basis_.p, n, coef, k, v, n, t, n, 1.0);
-    apply(1, v, cols, nullptr, metric_v, true);
+    // H v and S v in one pair application (one projection B v for both);
+    // the Cholesky normalisation T = v U^-1 takes S T = (S v) U^-1 and
+    // H T = (H v) U^-1 by the same triangular factor
+    double* hv = w3_.p + 4 * n * R;
+    apply(2, v, cols, hv, metric_v, true);
     double* gram = m3_.p;
     double* gh = m3_.p + 2 * cols * cols;
     gemm(true, 1, 0, cols, cols, n, 1.0, v, n, metric_v, n, gram, cols);
     check(launch_hermitise(gram, cols, cols, gh, cols, s_), "hermitise");
     long added = cols;
-    if (!cholesky_normalise(v, metric_v, gram, gh, cols, true)) {
+    bool h_images = false;
+    if (cholesky_normalise(v, metric_v, gram, gh, cols, true)) {
+      gemm(true, 0, 0, n, cols, cols, 1.0, hv, n, gh, cols, col(hb_, k, n), n);
+      h_images = true;
+    } else {
       // nearly dependent corrections (a correction inside span(W) up to its
       // search-arithmetic rounding): the reference's block CGS2 with its
       // eigen-normalisation and dependency test keeps the independent part
@@ -873,7 +895,7 @@ bool Solver::refine(long& it, long& iterations) {
         return false;
       }
     }
-    apply(0, col(basis_, k, n), added, col(hb_, k, n), nullptr, true);
+    if (!h_images) apply(0, col(basis_, k, n), added, col(hb_, k, n), nullptr, true);
     size_ = k + added;
     // 4. Rayleigh-Ritz of [W, T]
     const long m = size_;
