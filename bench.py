@@ -206,12 +206,14 @@ def main():
     ap.add_argument("--refinement", type=int, default=1,
                     help="FP32 search after the hand-off: 1 = block Jacobi-Davidson refinement (FP32 CG correction "
                          "solves, FP64 images and Rayleigh-Ritz); 0 = the reference's FP64 Davidson iterations")
-    ap.add_argument("--inner-iterations", type=int, default=8,
-                    help="CG steps per refinement correction solve (with 60 guards: 6 / 8 gave 766 / 780; with 160: "
-                         "4 / 6 / 8 gave 691 / 716 / 711)")
+    ap.add_argument("--inner-iterations", type=int, default=6,
+                    help="CG steps per refinement correction solve (with 60 guards and BF16 projector products: "
+                         "4 / 5 / 6 / 7 / 8 / 10 gave 1137 / 1172 / 1199-1201 / 1167 / 1173 / 1086 certified "
+                         "eigenpairs/s)")
     ap.add_argument("--refinement-guards", type=int, default=60,
-                    help="guard Ritz vectors beside the 160 roots in the refinement block (with 8 CG steps: "
-                         "32 / 60 / 80 / 160 gave 746 / 780 / 766 / 711; 0 = roots / 2)")
+                    help="guard Ritz vectors beside the 160 roots in the refinement block (with 6 CG steps: "
+                         "50 / 60 / 70 gave 1168 / 1199 / 1136; with 8: 40 / 60 / 80 gave 1146 / 1173 / 1148; "
+                         "0 = roots / 2)")
     ap.add_argument("--bf16-projectors", type=int, default=1,
                     help="1: the refinement's correction solves run their projector GEMMs on BF16 MFMA with FP32 "
                          "accumulation; 0: FP32 MFMA (1 / 0 gave 1056 / 1013 certified eigenpairs/s, 12 iterations "

@@ -85,7 +85,7 @@ def test_si64_davidson_against_reference(hip_ctx):
             dev.davidson(160, 256, 1e-12, init, subspace_factor=5, search_precision=1, refinement=1,
                          search_handoff=1.0, inner_iterations=8, refinement_guards=60),
             dev.davidson(160, 256, 1e-12, init, subspace_factor=5, search_precision=1, refinement=1,
-                         search_handoff=1.0, inner_iterations=8, refinement_guards=60, bf16_projectors=1)]
+                         search_handoff=1.0, inner_iterations=6, refinement_guards=60, bf16_projectors=1)]
     for r in runs:
         assert r["converged"]
         assert fresh_residuals(dev, r).max() <= 1.01e-12
