@@ -240,8 +240,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by k-point/request "
-              f"across GPUs with no exchange, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by k-point and by "
+              f"request across GPUs with no exchange between them", file=sys.stderr)
         sys.exit(2)
 
     from sdft_amd import native
