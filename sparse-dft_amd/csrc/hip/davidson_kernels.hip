@@ -454,29 +454,40 @@ __global__ __launch_bounds__(TPB) void k_davidson_seeds(long n, long warm_cols, 
     col[2 * i + 1] = seed_uniform(base + 2 * i + 1);
   }
   __syncthreads();
-  // the sums run in index order; the loads of 32 entries are issued ahead of
-  // their (dependent) additions
-  auto ordered_norm = [n](const double* v) {
-    constexpr int kAhead = 32;
+  // Each norm is summed in index order by one wave: its 64 lanes load 64
+  // consecutive entries and form re^2 + im^2 in parallel, then every lane adds
+  // the 64 terms in lane order from lane reads (the same terms in the same
+  // order as one thread walking the column, without its dependent loads).
+  auto ordered_norm = [n](const double* v, int lane) {
     double nn = 0.;
-    long i = 0;
-    for (; i + kAhead <= n; i += kAhead) {
-      double re[kAhead], im[kAhead];
-#pragma unroll
-      for (int t = 0; t < kAhead; ++t) {
-        re[t] = v[2 * (i + t)];
-        im[t] = v[2 * (i + t) + 1];
+    for (long i0 = 0; i0 < n; i0 += 64) {
+      const long i = i0 + lane;
+      double sq = 0.;
+      if (i < n) {
+        const double re = v[2 * i], im = v[2 * i + 1];
+        sq = re * re + im * im;
       }
+      const long m = n - i0 < 64 ? n - i0 : 64;
+      const unsigned long long bits = __builtin_bit_cast(unsigned long long, sq);
+      const int lo = int(bits & 0xffffffffull), hi = int(bits >> 32);
 #pragma unroll
-      for (int t = 0; t < kAhead; ++t) nn += re[t] * re[t] + im[t] * im[t];
+      for (int t = 0; t < 64; ++t) {
+        if (t >= m) break;
+        const unsigned long long tb =
+            (unsigned long long)(unsigned)__builtin_amdgcn_readlane(lo, t) |
+            ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, t) << 32);
+        nn += __builtin_bit_cast(double, tb);
+      }
     }
-    for (; i < n; ++i) nn += v[2 * i] * v[2 * i] + v[2 * i + 1] * v[2 * i + 1];
     return sqrt(nn);
   };
-  if (threadIdx.x == 0) {
-    norms[0] = ordered_norm(col);
-  } else if (threadIdx.x == 64 && warm) {
-    norms[1] = ordered_norm(src);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (wave == 0) {
+    const double r = ordered_norm(col, lane);
+    if (lane == 0) norms[0] = r;
+  } else if (wave == 1 && warm) {
+    const double r = ordered_norm(src, lane);
+    if (lane == 0) norms[1] = r;
   }
   __syncthreads();
   const double nb = norms[0];
