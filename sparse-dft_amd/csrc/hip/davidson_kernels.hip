@@ -411,6 +411,75 @@ __global__ void k_cg_update(double* __restrict__ t, double* __restrict__ r, cons
   }
 }
 
+// One pass of a CG step after its product q: alpha = rho / (p, q) (0 as in
+// k_cg_update), t += alpha p, r -= alpha q, z = M r with the kinetic
+// preconditioner of the column's root, and the partial sums of Re (r, z) over
+// the rows of this workgroup (kCgParts workgroups per column, rows split into
+// contiguous ranges, each summed in a fixed order): part[col * kCgParts + g].
+constexpr int kCgParts = 8;
+__global__ __launch_bounds__(256) void k_cg_step(double* __restrict__ t, double* __restrict__ r,
+                                                 const double* __restrict__ p, const double* __restrict__ q,
+                                                 double* __restrict__ z, long ld, long n,
+                                                 const double* __restrict__ rho, const double* __restrict__ pq,
+                                                 const int* __restrict__ src, const double* __restrict__ wk,
+                                                 const double* __restrict__ kinetic, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const long col = blockIdx.y;
+  const int g = blockIdx.x;
+  const double c = pq[col], num = rho[col];
+  const double a = (c > 0. && isfinite(c) && isfinite(num)) ? num / c : 0.;
+  const double w = wk[src[col]];
+  const long chunk = (n + kCgParts - 1) / kCgParts;
+  const long i0 = g * chunk, i1 = i0 + chunk < n ? i0 + chunk : n;
+  double acc = 0.;
+  for (long i = i0 + threadIdx.x; i < i1; i += 256) {
+    const long k = 2 * (i + col * ld);
+    double2 rv = *reinterpret_cast<const double2*>(r + k);
+    if (a != 0.) {
+      const double2 pv = *reinterpret_cast<const double2*>(p + k);
+      const double2 qv = *reinterpret_cast<const double2*>(q + k);
+      double2 tv = *reinterpret_cast<const double2*>(t + k);
+      tv.x += a * pv.x;
+      tv.y += a * pv.y;
+      rv.x -= a * qv.x;
+      rv.y -= a * qv.y;
+      *reinterpret_cast<double2*>(t + k) = tv;
+      *reinterpret_cast<double2*>(r + k) = rv;
+    }
+    const double kp = kinetic_preconditioner(kinetic[i], w);
+    const double2 zv{rv.x * kp, rv.y * kp};
+    *reinterpret_cast<double2*>(z + k) = zv;
+    acc += rv.x * zv.x + rv.y * zv.y;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[col * kCgParts + g] = sh[0];
+}
+
+// rho_new = the column's kCgParts partial sums in order; beta = rho_new /
+// rho_old (0 when rho_old is not positive); p = z + beta p; rho_new is stored
+// for the next step's alpha
+__global__ void k_cg_direction_parts(double* __restrict__ p, const double* __restrict__ z, long ld, long n,
+                                     const double* __restrict__ part, const double* __restrict__ rho_old,
+                                     double* __restrict__ rho_new) {
+  const long col = blockIdx.y;
+  double v = 0.;
+#pragma unroll
+  for (int g = 0; g < kCgParts; ++g) v += part[col * kCgParts + g];
+  const double o = rho_old[col];
+  const double b = (o > 0. && isfinite(o) && isfinite(v)) ? v / o : 0.;
+  if (blockIdx.x == 0 && threadIdx.x == 0) rho_new[col] = v;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long k = 2 * (i + col * ld);
+    p[k] = z[k] + b * p[k];
+    p[k + 1] = z[k + 1] + b * p[k + 1];
+  }
+}
+
 // beta = rho_new / rho_old (0 when rho_old is not positive); p = z + beta p
 __global__ void k_cg_direction(double* __restrict__ p, const double* __restrict__ z, long ld, long n,
                                const double* __restrict__ rho_new, const double* __restrict__ rho_old) {
@@ -429,7 +498,7 @@ __global__ void k_cg_direction(double* __restrict__ p, const double* __restrict_
 // the sequential host recipe: uniform() call c returns splitmix64 of
 // state0 + (c + 1) * golden, so every call is computed where it is needed;
 // column c of the block consumes calls [2 n c, 2 n (c + 1)) (its broad()
-// vector b). The two norms are summed by single threads in index order, as
+// vector b). The two norms are summed in index order (one wave each), as
 // the host loops do (this file is built without FMA contraction).
 //   warm column (c < roots when warm): src + (pert |src|) b / |b|
 //   guard column g: 0.05 b / |b|, plus 1 at row order[g]
@@ -567,6 +636,21 @@ hipError_t launch_cg_direction(double* p, const double* z, long ld, long n, long
   if (cols <= 0) return hipSuccess;
   SDFT_CHECKED(hipLaunchKernelGGL(k_cg_direction, grid_cols(n, cols), dim3(256), 0, s, p, z, ld, n, rho_new, rho_old));
 }
+hipError_t launch_cg_step(double* t, double* r, const double* p, const double* q, double* z, long ld, long n,
+                          long cols, const double* rho, const double* pq, const int* src, const double* wk,
+                          const double* kinetic, double* part, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  if (cols > 65535) return hipErrorInvalidValue;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_cg_step, dim3(kCgParts, unsigned(cols)), dim3(256), 0, s, t, r, p, q, z, ld, n,
+                                  rho, pq, src, wk, kinetic, part));
+}
+hipError_t launch_cg_direction_parts(double* p, const double* z, long ld, long n, long cols, const double* part,
+                                     const double* rho_old, double* rho_new, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_cg_direction_parts, grid_cols(n, cols), dim3(256), 0, s, p, z, ld, n, part,
+                                  rho_old, rho_new));
+}
+int cg_step_parts() { return kCgParts; }
 hipError_t launch_ritz_residual(const double* hx, const double* sx, long ld, long n, long cols, const double* values,
                                 double* r, hipStream_t s) {
   if (cols <= 0) return hipSuccess;

@@ -45,6 +45,15 @@ hipError_t launch_jd_precondition(const double* r, long ldr, const int* src, lon
 hipError_t launch_cg_update(double* t, double* r, const double* p, const double* q, long ld, long n, long cols,
                             const double* rho, const double* pq, hipStream_t s);
 // b = rho_new / rho_old per column (0 unless rho_old > 0); p = z + b p
+// fused CG step pass: alpha update of t and r, z = M r, partial sums of Re (r, z)
+// (cg_step_parts() per column, part: cols x cg_step_parts())
+hipError_t launch_cg_step(double* t, double* r, const double* p, const double* q, double* z, long ld, long n,
+                          long cols, const double* rho, const double* pq, const int* src, const double* wk,
+                          const double* kinetic, double* part, hipStream_t s);
+// rho_new = sum of the partials in order, p = z + (rho_new / rho_old) p
+hipError_t launch_cg_direction_parts(double* p, const double* z, long ld, long n, long cols, const double* part,
+                                     const double* rho_old, double* rho_new, hipStream_t s);
+int cg_step_parts();
 hipError_t launch_cg_direction(double* p, const double* z, long ld, long n, long cols, const double* rho_new,
                                const double* rho_old, hipStream_t s);
 hipError_t launch_finite_check(const double* x, long ld, long n, long cols, int* bad, hipStream_t s);

@@ -740,6 +740,8 @@ void Solver::correction_cg(const std::vector<int>& active, double* t) {
   double* rho1 = base + 3 * R;
   double* pq = base + 4 * R;
   int* src = reinterpret_cast<int*>(base + 5 * R);
+  double* part = base + 6 * R;  // cols x cg_step_parts() partial sums of (r, z)
+  if (long(cg_step_parts()) * cols > 8 * R) throw std::logic_error("HIP: CG partial sums exceed their buffer");
   std::vector<double> th(cols);
   for (long c = 0; c < cols; ++c) th[c] = values_[active[c]];
   upload(theta, th.data(), cols);
@@ -763,7 +765,14 @@ void Solver::correction_cg(const std::vector<int>& active, double* t) {
     stats_.h_columns_search += cols;
     project(q, true);
     check(launch_real_dots(p, n, q, n, n, cols, pq, s_), "dots");
-    check(launch_cg_update(t, res, p, q, n, n, cols, rho0, pq, s_), "cg update");
+    const bool last = it + 1 == steps;
+    // the update, the preconditioned residual z = M r and the partial sums of
+    // (r, z) in one pass; the last step only updates t and r
+    if (last)
+      check(launch_cg_update(t, res, p, q, n, n, cols, rho0, pq, s_), "cg update");
+    else
+      check(launch_cg_step(t, res, p, q, z, n, n, cols, rho0, pq, src, wk, op_.kinetic_device(), part, s_),
+            "cg step");
     if (p_.verbose > 1) {
       const std::vector<double> cur = col_norms(res, n, cols);
       const std::vector<double> curv = to_host(pq, cols);
@@ -775,15 +784,13 @@ void Solver::correction_cg(const std::vector<int>& active, double* t) {
       std::fprintf(stderr, "  cg step %d: max relative residual %.3e, nonpositive curvature %g of %ld\n", it, worst, neg,
                    cols);
     }
-    if (it + 1 == steps) break;
+    if (last) break;
     // the later preconditioned residuals are not projected again (one
     // projection per step instead of two): the first direction starts in the
     // range of P, and step 3 of refine() projects the correction against W
     // whatever the iterates picked up; measured, the solve takes as many
     // steps and 2 % less time
-    check(launch_jd_precondition(res, n, src, cols, wk, op_.kinetic_device(), n, z, n, s_), "precondition");
-    check(launch_real_dots(res, n, z, n, n, cols, rho1, s_), "dots");
-    check(launch_cg_direction(p, z, n, n, cols, rho1, rho0, s_), "cg direction");
+    check(launch_cg_direction_parts(p, z, n, n, cols, part, rho0, rho1, s_), "cg direction");
     std::swap(rho0, rho1);
   }
 }
