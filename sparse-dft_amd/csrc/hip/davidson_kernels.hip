@@ -228,6 +228,22 @@ __global__ void k_divide_columns(double* __restrict__ x, long ld, long n, const 
   }
 }
 
+// values = energy / metric and roots = sqrt(metric) per column (the Rayleigh
+// quotients and S norms of a block, left on the device)
+__global__ void k_rayleigh_quotients(const double* __restrict__ metric, const double* __restrict__ energy, long cols,
+                                     double* __restrict__ values, double* __restrict__ roots) {
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < cols; c += (long)gridDim.x * blockDim.x) {
+    values[c] = energy[c] / metric[c];
+    roots[c] = sqrt(metric[c]);
+  }
+}
+
+// divisors that are not positive and finite become 1 (zero columns stay zero)
+__global__ void k_guard_divisors(double* __restrict__ d, long cols) {
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < cols; c += (long)gridDim.x * blockDim.x)
+    if (!(d[c] > 0.) || !isfinite(d[c])) d[c] = 1.;
+}
+
 // trial(:, j) = fma(alpha_j, d(:, j), x(:, j)) where alpha_j != 0, else x
 __global__ void k_polish_update(const double* __restrict__ x, const double* __restrict__ d, long ld, long n,
                                 const double* __restrict__ alpha, double* __restrict__ t) {
@@ -664,6 +680,16 @@ hipError_t launch_fma_residual(const double* hx, const double* sx, long ld, long
 hipError_t launch_scale_columns(double* x, long ld, long n, long cols, const double* scales, hipStream_t s) {
   if (cols <= 0) return hipSuccess;
   SDFT_CHECKED(hipLaunchKernelGGL(k_scale_columns, grid_cols(n, cols), dim3(256), 0, s, x, ld, n, scales));
+}
+hipError_t launch_rayleigh_quotients(const double* metric, const double* energy, long cols, double* values,
+                                     double* roots, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_rayleigh_quotients, dim3(unsigned((cols + 255) / 256)), dim3(256), 0, s, metric,
+                                  energy, cols, values, roots));
+}
+hipError_t launch_guard_divisors(double* d, long cols, hipStream_t s) {
+  if (cols <= 0) return hipSuccess;
+  SDFT_CHECKED(hipLaunchKernelGGL(k_guard_divisors, dim3(unsigned((cols + 255) / 256)), dim3(256), 0, s, d, cols));
 }
 hipError_t launch_divide_columns(double* x, long ld, long n, long cols, const double* divisors, hipStream_t s) {
   if (cols <= 0) return hipSuccess;

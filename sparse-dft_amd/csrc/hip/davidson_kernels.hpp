@@ -26,6 +26,11 @@ hipError_t launch_ritz_residual(const double* hx, const double* sx, long ld, lon
 hipError_t launch_fma_residual(const double* hx, const double* sx, long ld, long n, long cols, const double* values,
                                double* r, hipStream_t s);
 hipError_t launch_scale_columns(double* x, long ld, long n, long cols, const double* scales, hipStream_t s);
+// values = energy / metric, roots = sqrt(metric), per column
+hipError_t launch_rayleigh_quotients(const double* metric, const double* energy, long cols, double* values,
+                                     double* roots, hipStream_t s);
+// d = d where positive and finite, else 1
+hipError_t launch_guard_divisors(double* d, long cols, hipStream_t s);
 hipError_t launch_divide_columns(double* x, long ld, long n, long cols, const double* divisors, hipStream_t s);
 hipError_t launch_polish_update(const double* x, const double* d, long ld, long n, long cols, const double* alpha,
                                 double* trial, hipStream_t s);

@@ -817,20 +817,31 @@ bool Solver::refine(long& it, long& iterations) {
     } else {
       apply(2, basis_.p, k, hb_.p, sb_.p, true);
     }
-    const std::vector<double> metric = real_dots(basis_.p, sb_.p, R);
-    const std::vector<double> energy = real_dots(basis_.p, hb_.p, R);
-    std::vector<double> root(R);
-    for (long r = 0; r < R; ++r) {
-      if (!(metric[r] > 0) || !std::isfinite(energy[r])) throw std::runtime_error("HIP: invalid refinement Ritz metric");
-      values_[r] = energy[r] / metric[r];
-      root[r] = std::sqrt(metric[r]);
+    // Rayleigh quotients, S-normalisation of X and its residuals on the
+    // device; metrics, energies and residual norms come back in one transfer
+    {
+      double* dm = small_.p + 8;
+      double* de = dm + R;
+      double* dv = de + R;
+      double* droot = dv + R;
+      double* dn = droot + R;
+      check(launch_real_dots(basis_.p, n, sb_.p, n, n, R, dm, s_), "dots");
+      check(launch_real_dots(basis_.p, n, hb_.p, n, n, R, de, s_), "dots");
+      check(launch_rayleigh_quotients(dm, de, R, dv, droot, s_), "Rayleigh quotients");
+      for (DevBuf<double>* b : {&basis_, &hb_, &sb_})
+        check(launch_divide_columns(b->p, n, n, R, droot, s_), "normalise");
+      check(launch_fma_residual(hb_.p, sb_.p, n, n, R, dv, res_.p, s_), "residual");
+      check(launch_col_norms(res_.p, n, n, R, dn, s_), "norms");
+      const std::vector<double> h = to_host(dm, 5 * R);
+      for (long r = 0; r < R; ++r) {
+        const double metric = h[r], energy = h[R + r];
+        if (!(metric > 0) || !std::isfinite(energy)) throw std::runtime_error("HIP: invalid refinement Ritz metric");
+        values_[r] = energy / metric;
+        res_norms_[r] = h[4 * R + r];
+      }
+      converged_ = true;
+      for (long r = 0; r < R; ++r) converged_ = converged_ && res_norms_[r] <= tol_;
     }
-    upload(small_.p + 8, root.data(), R);
-    for (DevBuf<double>* b : {&basis_, &hb_, &sb_})
-      check(launch_divide_columns(b->p, n, n, R, small_.p + 8, s_), "normalise");
-    upload(small_.p + 8 + R, values_.data(), R);
-    check(launch_fma_residual(hb_.p, sb_.p, n, n, R, small_.p + 8 + R, res_.p, s_), "residual");
-    refresh_norms();
     stats_.max_residual_history.push_back(max_res());
     if (search_residual_ >= 0) {
       if (max_res() > std::max(8 * tol_, 10 * search_residual_)) ++stats_.search_rejected;
@@ -862,13 +873,9 @@ bool Solver::refine(long& it, long& iterations) {
     // the corrections scale with their roots' residuals, which span many
     // orders of magnitude once some roots converge; unit columns keep the
     // Cholesky pivot test below a test of linear dependence, not of scale
-    {
-      std::vector<double> norms = col_norms(t, n, cols);
-      for (double& v : norms)
-        if (!(v > 0) || !std::isfinite(v)) v = 1.0;
-      upload(small_.p + 8, norms.data(), cols);
-      check(launch_divide_columns(t, n, n, cols, small_.p + 8, s_), "unit corrections");
-    }
+    check(launch_col_norms(t, n, n, cols, small_.p + 8, s_), "norms");
+    check(launch_guard_divisors(small_.p + 8, cols, s_), "norms");
+    check(launch_divide_columns(t, n, n, cols, small_.p + 8, s_), "unit corrections");
     // 3. T projected against W and S-orthonormalised, with FP64 images
     size_ = k;
     double* v = w3_.p;
