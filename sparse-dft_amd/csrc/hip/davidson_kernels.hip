@@ -539,30 +539,43 @@ __global__ __launch_bounds__(TPB) void k_davidson_seeds(long n, long warm_cols, 
     col[2 * i + 1] = seed_uniform(base + 2 * i + 1);
   }
   __syncthreads();
-  // Each norm is summed in index order by one wave: its 64 lanes load 64
-  // consecutive entries and form re^2 + im^2 in parallel, then every lane adds
-  // the 64 terms in lane order from lane reads (the same terms in the same
-  // order as one thread walking the column, without its dependent loads).
+  // Each norm is summed in index order by one wave: its 64 lanes form
+  // re^2 + im^2 of 64 consecutive entries in parallel, then every lane adds the
+  // 64 terms in lane order from lane reads (the same terms in the same order as
+  // one thread walking the column; entries past n contribute +0, which leaves
+  // the non-negative sum unchanged bit for bit). Loads use clamped indices
+  // without branches, and the next 4 x 64 entries are loaded while the current
+  // ones are summed.
   auto ordered_norm = [n](const double* v, int lane) {
-    double nn = 0.;
-    for (long i0 = 0; i0 < n; i0 += 64) {
-      const long i = i0 + lane;
-      double sq = 0.;
-      if (i < n) {
-        const double re = v[2 * i], im = v[2 * i + 1];
-        sq = re * re + im * im;
-      }
-      const long m = n - i0 < 64 ? n - i0 : 64;
-      const unsigned long long bits = __builtin_bit_cast(unsigned long long, sq);
-      const int lo = int(bits & 0xffffffffull), hi = int(bits >> 32);
+    constexpr int D = 4;
+    auto fetch = [&](long i0, double* sq) {
 #pragma unroll
-      for (int t = 0; t < 64; ++t) {
-        if (t >= m) break;
-        const unsigned long long tb =
-            (unsigned long long)(unsigned)__builtin_amdgcn_readlane(lo, t) |
-            ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, t) << 32);
-        nn += __builtin_bit_cast(double, tb);
+      for (int u = 0; u < D; ++u) {
+        const long i = i0 + 64 * u + lane;
+        const long ic = i < n ? i : n - 1;
+        const double2 z = *reinterpret_cast<const double2*>(v + 2 * ic);
+        const double m = z.x * z.x + z.y * z.y;
+        sq[u] = i < n ? m : 0.;
       }
+    };
+    double nn = 0., cur[D], nxt[D] = {0., 0., 0., 0.};
+    fetch(0, cur);
+    for (long i0 = 0; i0 < n; i0 += 64 * D) {
+      if (i0 + 64 * D < n) fetch(i0 + 64 * D, nxt);
+#pragma unroll
+      for (int u = 0; u < D; ++u) {
+        const unsigned long long bits = __builtin_bit_cast(unsigned long long, cur[u]);
+        const int lo = int(bits & 0xffffffffull), hi = int(bits >> 32);
+#pragma unroll
+        for (int t = 0; t < 64; ++t) {
+          const unsigned long long tb =
+              (unsigned long long)(unsigned)__builtin_amdgcn_readlane(lo, t) |
+              ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, t) << 32);
+          nn += __builtin_bit_cast(double, tb);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < D; ++u) cur[u] = nxt[u];
     }
     return sqrt(nn);
   };
