@@ -228,6 +228,11 @@ typedef struct {
    * the correction equations are solved to a few digits only), 0 on FP32 MFMA.
    * Rayleigh-Ritz, images and the certificate are unaffected. */
   int bf16_projectors;
+  /* band groups: 1 runs the swept variant (SDFT_BAND_GROUP_SWEEP=1,
+   * src/band_group_solver.cpp:79-100, 197-208): local solves of at most 3
+   * iterations over max_iterations / 2 worth of sweeps, then the Davidson over
+   * all roots from the swept block if it is not certified */
+  int band_group_sweep;
 } sdft_hip_davidson_params;
 
 typedef struct {

@@ -591,6 +591,9 @@ int sdft_hip_davidson(sdft_hip_operator* op, const sdft_hip_davidson_params* par
     p.verbose = params->verbose;
     p.timing = params->timing;
     p.band_group_size = params->band_group_size;
+    if (params->band_group_sweep != 0 && params->band_group_sweep != 1)
+      throw std::invalid_argument("HIP: band_group_sweep must be 0 or 1");
+    p.band_group_sweep = params->band_group_sweep;
     p.refinement = params->refinement;
     p.inner_iterations = params->inner_iterations;
     p.refinement_guards = params->refinement_guards;

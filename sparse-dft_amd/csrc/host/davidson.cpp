@@ -1264,8 +1264,13 @@ DavidsonOutput davidson(DeviceOperator& op, const DavidsonParams& params, const 
   return s.run(initial);
 }
 
-// src/band_group_solver.cpp:57-211 (default: two sweeps, each group solved to
-// the full iteration limit, no image reuse).
+// src/band_group_solver.cpp:57-211 (no image reuse). Default: two sweeps,
+// each group solved to the full iteration limit. Swept (band_group_sweep,
+// the reference's SDFT_BAND_GROUP_SWEEP, :79-100 and :197-208): half the
+// iteration budget goes to sweeps whose local solves stop after 3 iterations
+// (budget / (groups x 3) sweeps; too small a budget solves all roots with the
+// Davidson at once), and a block the sweeps leave uncertified is refined by
+// the Davidson over all roots with the iterations that remain.
 DavidsonOutput davidson_band_groups(DeviceOperator& op, const DavidsonParams& params,
                                     const std::complex<double>* initial) {
   using C = std::complex<double>;
@@ -1274,6 +1279,25 @@ DavidsonOutput davidson_band_groups(DeviceOperator& op, const DavidsonParams& pa
       params.tolerance <= 0)
     throw std::invalid_argument("invalid bounded band-group problem");
   if (G < 1 || G > 64) throw std::invalid_argument("SDFT_BAND_GROUP_SIZE must be in [1,64]");
+  const bool swept = params.band_group_sweep != 0;
+  const long local_limit = 3, groups = (R + G - 1) / G, local_budget = params.max_iterations / 2;
+  // the Davidson over all roots, as a local solve (no strict polish)
+  auto whole_block = [&](const std::complex<double>* seed, long iterations, bool seed_on_device) {
+    DavidsonParams lp = params;
+    lp.band_group_size = 0;
+    lp.band_group_sweep = 0;
+    lp.band_group_local = 1;
+    lp.max_iterations = iterations;
+    if (seed && seed_on_device && !params.device_pointers) {
+      // the swept block lives on the device; a host caller gets host vectors back
+      std::vector<C> host(std::size_t(n) * R);
+      check(hipMemcpy(host.data(), seed, 16 * std::size_t(n) * R, hipMemcpyDeviceToHost), "seed");
+      return davidson(op, lp, host.data());
+    }
+    return davidson(op, lp, seed);
+  };
+  if (swept && groups * local_limit > local_budget) return whole_block(initial, params.max_iterations, false);
+  const long sweep_limit = swept ? local_budget / (groups * local_limit) : 2;
   Context& ctx = op.context();
   hipStream_t s = ctx.stream();
   const double t_start = now();
@@ -1317,12 +1341,13 @@ DavidsonOutput davidson_band_groups(DeviceOperator& op, const DavidsonParams& pa
   out.eigenvalues.assign(R, 0.);
   out.residual_norms.assign(R, 0.);
   bool converged = false;
-  for (int sweep = 0; sweep < 2 && !converged; ++sweep) {
+  for (long sweep = 0; sweep < sweep_limit && !converged; ++sweep) {
     for (long first = 0; first < R; first += G) {
       const long count = std::min(G, R - first);
       DavidsonParams lp = params;
       lp.roots = count;
       lp.band_group_local = 1;
+      if (swept) lp.max_iterations = local_limit;
       lp.vectors_out = nullptr;
       lp.device_pointers = 0;  // the group's seed and result pass through host vectors
       lp.capacity = std::min(n, 5 * G);
@@ -1351,7 +1376,7 @@ DavidsonOutput davidson_band_groups(DeviceOperator& op, const DavidsonParams& pa
       }
       const DavidsonOutput r = davidson(op, lp, seed.empty() ? nullptr : seed.data());
       if (params.verbose)
-        std::fprintf(stderr, "band_group sweep=%d first=%ld count=%ld iterations=%ld converged=%d\n", sweep, first,
+        std::fprintf(stderr, "band_group sweep=%ld first=%ld count=%ld iterations=%ld converged=%d\n", sweep, first,
                      count, r.stats.iterations, int(r.stats.converged));
       out.stats.iterations += r.stats.iterations;
       out.stats.h_columns_exact += r.stats.h_columns_exact;
@@ -1419,6 +1444,13 @@ DavidsonOutput davidson_band_groups(DeviceOperator& op, const DavidsonParams& pa
     out.residual_norms = to_host(o, R);
     converged = true;
     for (double r : out.residual_norms) converged = converged && r <= params.tolerance;
+  }
+  if (swept && !converged) {
+    // global refinement from the swept block (src/band_group_solver.cpp:197-208)
+    const long search_steps = out.stats.iterations;
+    DavidsonOutput refined = whole_block(reinterpret_cast<const C*>(x.p), params.max_iterations - search_steps, true);
+    refined.stats.iterations += search_steps;
+    return refined;
   }
   std::complex<double>* dst = params.vectors_out;
   if (!dst) {

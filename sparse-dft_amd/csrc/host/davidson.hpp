@@ -31,6 +31,7 @@ struct DavidsonParams {
   int verbose = 0;
   int timing = 0;                 // synchronise phases and accumulate per-phase times
   long band_group_size = 0;       // > 0: bounded band-group solver (src/band_group_solver.cpp:57-211)
+  int band_group_sweep = 0;       // band groups: 1 = the swept variant (SDFT_BAND_GROUP_SWEEP, :79-100, 197-208)
   // Local solve of one band group (src/band_group_solver.cpp:84-100): capacity
   // override, completed bands to keep out of the search space (S-orthonormal,
   // device n x excluded_count), and no strict polish (src/pw_core.cpp:1273).

@@ -38,7 +38,7 @@ class DavidsonParams(ct.Structure):
                 ("stagnation_window", ct.c_int), ("handoff_restart", ct.c_int), ("verbose", ct.c_int),
                 ("timing", ct.c_int), ("band_group_size", ct.c_long), ("refinement", ct.c_int),
                 ("inner_iterations", ct.c_int), ("refinement_guards", ct.c_long), ("device_pointers", ct.c_int),
-                ("bf16_projectors", ct.c_int)]
+                ("bf16_projectors", ct.c_int), ("band_group_sweep", ct.c_int)]
 
 
 class DavidsonResult(ct.Structure):
@@ -627,7 +627,7 @@ class Operator:
     def davidson(self, roots, max_iterations=256, tolerance=1e-12, initial=None, subspace_factor=8,
                  search_precision=0, search_handoff=1e-8, stagnation_window=3, handoff_restart=1, verbose=0,
                  timing=0, want_vectors=True, band_group_size=0, refinement=0, inner_iterations=0,
-                 refinement_guards=0, device_blocks=None, bf16_projectors=0):
+                 refinement_guards=0, device_blocks=None, bf16_projectors=0, band_group_sweep=0):
         """device_blocks = (initial, eigenvectors): device addresses of n x
         roots complex128 column-major blocks resident in HBM (initial 0 for a
         cold start); the solve then moves no block over PCIe and returns
@@ -635,7 +635,7 @@ class Operator:
         p = DavidsonParams(roots, max_iterations, tolerance, subspace_factor, search_precision, search_handoff,
                            stagnation_window, handoff_restart, verbose, timing, band_group_size, refinement,
                            inner_iterations, refinement_guards, 1 if device_blocks is not None else 0,
-                           bf16_projectors)
+                           bf16_projectors, band_group_sweep)
         ev = np.zeros(roots)
         res = np.zeros(roots)
         r = DavidsonResult()

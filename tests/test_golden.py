@@ -100,3 +100,17 @@ def test_large_golden_fixture(name, config, kfrac, roots):
     ev = LARGE[f"{name}_eigenvalues"]
     assert ev.shape == (roots,) and np.all(np.diff(ev) >= 0)
     assert float(LARGE[f"{name}_max_residual"]) <= 1e-12 and float(LARGE[f"{name}_fresh_residual"]) <= 1e-12
+
+
+def test_swept_band_group_fixture_consistent():
+    """The reference's swept band-group solver (band_sweep_golden.npz) finds the
+    same 16 roots as its default band groups and its Davidson
+    (reference_golden.npz), certified."""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sweep = np.load(os.path.join(here, "band_sweep_golden.npz"))
+    ref = np.load(os.path.join(here, "reference_golden.npz"))
+    for g in (8, 5):
+        assert np.abs(sweep[f"si8_bandsweep{g}_eigenvalues"] - ref["si8_davidson16_eigenvalues"]).max() < 1e-10
+        assert float(sweep[f"si8_bandsweep{g}_max_residual"]) <= 1e-12
+        assert 0 < int(sweep[f"si8_bandsweep{g}_iterations"]) <= 256
